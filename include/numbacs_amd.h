@@ -1,0 +1,152 @@
+/*
+ * numbacs_amd.h -- C ABI of libnumbacs_amd.so, the MI355X (gfx950) FTLE hot
+ * path: DOP853 particle flow maps, FTLE / Cauchy-Green eigen stencils and
+ * FTLE ridge extraction.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer unless its name ends in _host.
+ *   - Arrays use the reference's 'ij' C-order layouts: a flow map is
+ *     (nx, ny, 2) float64, an FTLE field (nx, ny) float64, eigenvalues
+ *     (nx, ny, 2), eigenvectors (nx, ny, 2, 2) with eigenvectors as columns.
+ *   - Every launch is enqueued on the given stream (NULL = default stream);
+ *     a call returns once its work is enqueued unless it says otherwise.
+ *   - Return value 0 = success, < 0 = error; nbx_last_error() then returns a
+ *     thread-local message.
+ *
+ * The reference's own FFI for this path is numbalsoda's
+ *   dop853(funcptr, u0, t_eval, rtol, atol, data=params)
+ * called once per particle with funcptr = address of a numba cfunc
+ *   void rhs(double t, double* y, double* dy, double* p)
+ * (integration.py:169-171, flows.py:1146).  Here the cfunc address is
+ * replaced by an opaque nbx_flow handle and the per-particle loop by one
+ * grid-wide launch.
+ */
+#ifndef NUMBACS_AMD_H
+#define NUMBACS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nbx_flow nbx_flow;
+typedef void* nbx_stream; /* a hipStream_t */
+
+enum nbx_flow_kind { NBX_FLOW_DOUBLE_GYRE = 0, NBX_FLOW_BICKLEY_JET = 1, NBX_FLOW_SPLINE3 = 2 };
+enum nbx_extrap { NBX_EXTRAP_CONSTANT = 0, NBX_EXTRAP_LINEAR = 1, NBX_EXTRAP_NEAREST = 2 };
+enum nbx_precision { NBX_FP64 = 0, NBX_FP32_DISP = 1 };
+
+/* Per-particle status written by the flow-map launches (Hairer's idid). */
+enum nbx_status {
+    NBX_ST_MASKED = 0,      /* masked point, not integrated (output 0)          */
+    NBX_ST_OK = 1,          /* reached t0+T                                      */
+    NBX_ST_MAX_STEPS = -2,  /* step budget exhausted                             */
+    NBX_ST_H_SMALL = -3     /* step size underflow (0.1|h| <= |t| * 2.3e-16)     */
+};
+
+const char* nbx_last_error(void);
+int nbx_version(void);
+
+/* --- flows ------------------------------------------------------------- */
+
+/* An analytic predefined flow (get_predefined_flow, flows.py:1104-1245).
+ * Parameters are passed per launch (params[0] = integration direction). */
+int nbx_flow_create_analytic(int kind, nbx_flow** out);
+
+/* A gridded flow evaluated by a tricubic natural B-spline (get_flow_2D,
+ * flows.py:121-258).  grid9_host = {t0,t1,nt, x0,x1,nx, y0,y1,ny} (the
+ * reference's UCGrid tuple), Cu/Cv = DEVICE coefficient arrays of shape
+ * (nt+2, nx+2, ny+2) float64 (get_interp_arrays_2D's output layout).
+ * The handle keeps its own interleaved (u, v) copy in HBM.  spherical:
+ * 0 planar, 1 lon in [-180,180), 2 lon in [0,360) (degrees, km/h velocities
+ * converted with radius r). */
+int nbx_flow_create_spline3(const double* grid9_host, const double* Cu, const double* Cv,
+                            int spherical, int extrap, double r, nbx_flow** out,
+                            nbx_stream stream);
+void nbx_flow_destroy(nbx_flow* flow);
+
+/* Natural cubic B-spline prefilter (get_interp_arrays_2D, flows.py:39-46):
+ * U (nt, nx, ny) -> C (nt+2, nx+2, ny+2). */
+int nbx_prefilter3(const double* U, int nt, int nx, int ny, double* C, nbx_stream stream);
+
+/* Velocity samples: out[k] = rhs(t[k], pts[k]) for k < n (pts (n,2)). */
+int nbx_flow_eval(const nbx_flow* flow, const double* params_host, int nparams, const double* t,
+                  const double* pts, int64_t n, double* out, nbx_stream stream);
+
+/* --- flow maps --------------------------------------------------------- */
+
+/* flowmap_grid_2D (integration.py:123-182) with method="dop853": one
+ * adaptive DOP853 solve per (x[i], y[j]) from s0 = params[0]*t0 to
+ * s1 = params[0]*(t0+T); out (nx, ny, 2).  mask (nx*ny bytes, nonzero =
+ * skip, may be NULL) leaves 0 in out.  status / nsteps (nx*ny int32, may be
+ * NULL) receive nbx_status and the attempted step count.  max_steps bounds
+ * every solve (Hairer's NMAX).  precision selects FP64 or the
+ * FP32-displacement form (FP32 state + spline, FP64 step control). */
+int nbx_flowmap_grid_2d(const nbx_flow* flow, double t0, double T, const double* x, int nx,
+                        const double* y, int ny, const double* params_host, int nparams,
+                        double rtol, double atol, const uint8_t* mask, int64_t max_steps,
+                        int precision, double* out, int32_t* status, int32_t* nsteps,
+                        nbx_stream stream);
+
+/* flowmap (integration.py:7-61): pts (npts, 2) -> out (npts, 2). */
+int nbx_flowmap_pts(const nbx_flow* flow, double t0, double T, const double* pts, int64_t npts,
+                    const double* params_host, int nparams, double rtol, double atol,
+                    const uint8_t* mask, int64_t max_steps, int precision, double* out,
+                    int32_t* status, int32_t* nsteps, nbx_stream stream);
+
+/* --- FTLE and Cauchy-Green eigen --------------------------------------- */
+
+/* ftle_grid_2D (diagnostics.py:21-65): interior central differences,
+ * borders and masked points 0, ftle = log(lmax)/(2|T|) where lmax > 1. */
+int nbx_ftle_grid_2d(const double* fm, int nx, int ny, double T, double dx, double dy,
+                     const uint8_t* mask, double* ftle, nbx_stream stream);
+
+/* C_eig_2D (diagnostics.py:200-244): eigvals (nx,ny,2) ascending, eigvecs
+ * (nx,ny,2,2) with eigvecs[i,j,:,k] the k-th eigenvector; zeros on the
+ * border and at masked points. */
+int nbx_C_eig_2d(const double* fm, int nx, int ny, double dx, double dy, const uint8_t* mask,
+                 double* eigvals, double* eigvecs, nbx_stream stream);
+
+/* Fused ridge-path stencil: C_eig_2D + ftle_from_eig (diagnostics.py:247-269)
+ * in one pass; writes ftle (nx,ny) and eigvec_max (nx,ny,2). */
+int nbx_ftle_eig_2d(const double* fm, int nx, int ny, double T, double dx, double dy,
+                    const uint8_t* mask, double* ftle, double* eigvec_max, nbx_stream stream);
+
+/* --- ridges ------------------------------------------------------------ */
+
+/* numpy.percentile(f, q) (linear method, ridges.py:51) of n values; the
+ * result is left in DEVICE memory at *result (1 double). */
+int nbx_percentile(const double* f, int64_t n, double q, double* result, nbx_stream stream);
+
+/* ftle_ridge_pts (ridges.py:9-76).  dx = x[1]-x[0] and dy = y[1]-y[0] as in
+ * the reference.  pts receives the ridge points (capacity nx*ny*2) in raveled
+ * (i*ny+j) order; *npts_host their number.  Synchronises the stream. */
+int nbx_ridge_pts(const double* f, const double* eigvec_max, const double* x, int nx,
+                  const double* y, int ny, double dx, double dy, double sdd_thresh,
+                  double percentile, double* pts, int64_t* npts_host, nbx_stream stream);
+
+/* ftle_ridges (ridges.py:186-229): ridge points grouped into 8-connected
+ * ridges (labels in raster order of each ridge's first point, points in
+ * raveled order, ridges with fewer than min_ridge_pts points dropped).
+ * pts (capacity nx*ny*2) holds the kept points ridge after ridge;
+ * offsets (capacity nx*ny+1 int64) the start of each ridge plus the end.
+ * *nridges_host and *npts_host receive the counts.  Synchronises the
+ * stream. */
+int nbx_ridges(const double* f, const double* eigvec_max, const double* x, int nx,
+               const double* y, int ny, double dx, double dy, double sdd_thresh,
+               double percentile, int min_ridge_pts, double* pts, int64_t* offsets,
+               int64_t* nridges_host, int64_t* npts_host, nbx_stream stream);
+
+/* 8-connected labelling of a byte grid (scipy.ndimage.label with a full
+ * 3x3 structure, ridges.py:221-222): labels (nx*ny int32, 0 = background,
+ * 1.. in raster order of first pixel); *nlabels_host the count.
+ * Synchronises the stream. */
+int nbx_label8(const uint8_t* b, int nx, int ny, int32_t* labels, int32_t* nlabels_host,
+               nbx_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUMBACS_AMD_H */

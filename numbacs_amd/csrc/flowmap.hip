@@ -1,0 +1,462 @@
+// Flow handles, spline prefilter and the DOP853 flow-map launches.
+//
+// Reference path: flowmap_grid_2D (integration.py:123-182) runs one
+// numbalsoda.dop853 solve per grid point over a numba cfunc right-hand side.
+// Here one GPU thread owns one particle: the right-hand side is a device
+// functor inlined into the integrator, the state stays in registers for the
+// whole solve, and the final position is written once, coalesced.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "dop853.h"
+#include "flows.h"
+
+struct nbx_flow {
+    int kind;
+    // spline flows
+    double2* coef = nullptr;  // (nt+2, nx+2, ny+2) interleaved (u, v), device
+    float2* coef32 = nullptr; // FP32 copy for the FP32-displacement path
+    int64_t ncoef = 0;
+    double grid9[9] = {0};
+    int spherical = 0;
+    int extrap = 0;
+    double r = 6371.0;
+};
+
+namespace nbx {
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_ws_mutex;
+struct WsSlot {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::vector<std::vector<WsSlot>> g_ws;  // [device][slot]
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+void* workspace(int slot, size_t bytes, int* err) {
+    *err = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        *err = -1;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    if (static_cast<int>(g_ws.size()) <= dev) g_ws.resize(dev + 1);
+    auto& slots = g_ws[dev];
+    if (static_cast<int>(slots.size()) <= slot) slots.resize(slot + 1);
+    WsSlot& s = slots[slot];
+    if (s.bytes < bytes) {
+        if (s.ptr) {
+            // hipFree synchronises the device, so no queued kernel still uses it
+            (void)hipFree(s.ptr);
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        size_t want = bytes < 4096 ? 4096 : bytes;
+        if (hipMalloc(&s.ptr, want) != hipSuccess) {
+            set_error("workspace hipMalloc failed");
+            *err = -1;
+            return nullptr;
+        }
+        s.bytes = want;
+    }
+    return s.ptr;
+}
+
+// ---------------------------------------------------------------------------
+// Spline prefilter: natural cubic B-spline, one thread per line.
+// Per axis n samples -> n+2 coefficients c[-1..n]: (c[i-1] + 4c[i] + c[i+1])/6
+// = f[i], zero second derivative at both ends, i.e. c[0] = f[0],
+// c[n-1] = f[n-1], interior by the Thomas algorithm, then
+// c[-1] = 2c[0]-c[1], c[n] = 2c[n-1]-c[n-2].
+
+__device__ void prefilter_line(double* p, long stride, int n) {
+    auto C = [&](int k) -> double& { return p[(static_cast<long>(k) + 1) * stride]; };
+    if (n == 1) {
+        C(-1) = C(0);
+        C(1) = C(0);
+        return;
+    }
+    if (n > 2) {
+        // forward sweep; the modified super-diagonal cp[k] = 1/(4 - cp[k-1])
+        // is recomputed on the way back instead of stored
+        const int m = n - 2;
+        double cp = 0.0, prev_d = 0.0;
+        for (int k = 0; k < m; ++k) {
+            const int i = k + 1;
+            double rhs = 6.0 * C(i);
+            if (i == 1) rhs -= C(0);
+            if (i == n - 2) rhs -= C(n - 1);
+            const double denom = 4.0 - (k > 0 ? cp : 0.0);
+            cp = 1.0 / denom;
+            const double d = (rhs - (k > 0 ? prev_d : 0.0)) / denom;
+            C(i) = d;
+            prev_d = d;
+        }
+        // back substitution needs cp[k] for k = m-2 .. 0.  The recurrence
+        // cp[k] = 1/(4 - cp[k-1]) reaches its double-precision fixed point
+        // (2 - sqrt(3)) by k = 13 and stays there, so 32 stored terms give
+        // every cp[k] exactly.
+        constexpr int kLocal = 32;
+        double cps[kLocal];
+        double c = 0.0;
+        for (int k = 0; k < kLocal; ++k) {
+            c = 1.0 / (4.0 - (k > 0 ? c : 0.0));
+            cps[k] = c;
+        }
+        for (int k = m - 2; k >= 0; --k)
+            C(k + 1) = C(k + 1) - cps[k < kLocal ? k : kLocal - 1] * C(k + 2);
+    }
+    C(-1) = 2 * C(0) - C(1);
+    C(n) = 2 * C(n - 1) - C(n - 2);
+}
+
+__global__ void k_prefilter_scatter(const double* __restrict__ U, int nt, int nx, int ny,
+                                    double* __restrict__ C) {
+    const long n2 = ny + 2, n1 = static_cast<long>(nx + 2) * n2;
+    const long total = static_cast<long>(nt + 2) * n1;
+    for (long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; k < total;
+         k += static_cast<long>(gridDim.x) * blockDim.x) {
+        const int a = static_cast<int>(k / n1);
+        const int b = static_cast<int>((k / n2) % (nx + 2));
+        const int c = static_cast<int>(k % n2);
+        double v = 0.0;
+        if (a >= 1 && a <= nt && b >= 1 && b <= nx && c >= 1 && c <= ny)
+            v = U[(static_cast<long>(a - 1) * nx + (b - 1)) * ny + (c - 1)];
+        C[k] = v;
+    }
+}
+
+// axis: 2 = y lines (a in 1..nt, b in 1..nx), 1 = x lines (a in 1..nt,
+// c in 0..ny+1), 0 = t lines (b in 0..nx+1, c in 0..ny+1)
+__global__ void k_prefilter_axis(double* __restrict__ C, int nt, int nx, int ny, int axis) {
+    const long n2 = ny + 2, n1 = static_cast<long>(nx + 2) * n2;
+    long lines, stride;
+    int n;
+    if (axis == 2) { lines = static_cast<long>(nt) * nx; stride = 1; n = ny; }
+    else if (axis == 1) { lines = static_cast<long>(nt) * (ny + 2); stride = n2; n = nx; }
+    else { lines = static_cast<long>(nx + 2) * (ny + 2); stride = n1; n = nt; }
+    const long L = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    if (L >= lines) return;
+    double* p;
+    if (axis == 2) {
+        const long a = 1 + L / nx, b = 1 + L % nx;
+        p = C + a * n1 + b * n2;
+    } else if (axis == 1) {
+        const long a = 1 + L / (ny + 2), c = L % (ny + 2);
+        p = C + a * n1 + c;
+    } else {
+        const long b = L / (ny + 2), c = L % (ny + 2);
+        p = C + b * n2 + c;
+    }
+    prefilter_line(p, stride, n);
+}
+
+__global__ void k_interleave(const double* __restrict__ u, const double* __restrict__ v, int64_t n,
+                             double2* __restrict__ out, float2* __restrict__ out32) {
+    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double a = u[k], b = v[k];
+        out[k] = make_double2(a, b);
+        out32[k] = make_float2(static_cast<float>(a), static_cast<float>(b));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Flow-map kernels.
+
+template <class Flow>
+__global__ __launch_bounds__(256) void k_flowmap_grid(Flow f, const double* __restrict__ xs, int nx,
+                                                      const double* __restrict__ ys, int ny,
+                                                      double s0, double s1, double rtol,
+                                                      double atol,
+                                                      const uint8_t* __restrict__ mask,
+                                                      long long nmax, double2* __restrict__ out,
+                                                      int32_t* __restrict__ status,
+                                                      int32_t* __restrict__ nsteps) {
+    const long long p = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (p >= static_cast<long long>(nx) * ny) return;
+    const int i = static_cast<int>(p / ny);
+    const int j = static_cast<int>(p - static_cast<long long>(i) * ny);
+    if (mask != nullptr && mask[p]) {
+        out[p] = make_double2(0.0, 0.0);
+        if (status) status[p] = NBX_ST_MASKED;
+        if (nsteps) nsteps[p] = 0;
+        return;
+    }
+    double a = xs[i], b = ys[j];
+    const SolveResult r = dop853_solve(f, s0, s1, a, b, rtol, atol, nmax);
+    out[p] = make_double2(a, b);
+    if (status) status[p] = r.idid;
+    if (nsteps) nsteps[p] = r.nstep;
+}
+
+template <class Flow>
+__global__ __launch_bounds__(256) void k_flowmap_pts(Flow f, const double2* __restrict__ pts,
+                                                     int64_t npts, double s0, double s1,
+                                                     double rtol, double atol,
+                                                     const uint8_t* __restrict__ mask,
+                                                     long long nmax, double2* __restrict__ out,
+                                                     int32_t* __restrict__ status,
+                                                     int32_t* __restrict__ nsteps) {
+    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (p >= npts) return;
+    if (mask != nullptr && mask[p]) {
+        out[p] = make_double2(0.0, 0.0);
+        if (status) status[p] = NBX_ST_MASKED;
+        if (nsteps) nsteps[p] = 0;
+        return;
+    }
+    double2 q = pts[p];
+    const SolveResult r = dop853_solve(f, s0, s1, q.x, q.y, rtol, atol, nmax);
+    out[p] = q;
+    if (status) status[p] = r.idid;
+    if (nsteps) nsteps[p] = r.nstep;
+}
+
+template <class Flow>
+__global__ void k_flow_eval(Flow f, const double* __restrict__ t, const double2* __restrict__ pts,
+                            int64_t n, double2* __restrict__ out) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k >= n) return;
+    double a, b;
+    f(t[k], pts[k].x, pts[k].y, a, b);
+    out[k] = make_double2(a, b);
+}
+
+// Build the device functor of a flow for one launch.
+int make_analytic(const nbx_flow* fl, const double* params, int np, DoubleGyre* dg,
+                  BickleyJet* bj) {
+    if (fl->kind == NBX_FLOW_DOUBLE_GYRE) {
+        NBX_REQUIRE(np >= 6, "double_gyre needs 6 parameters [dir, A, eps, alpha, omega, psi]");
+        *dg = DoubleGyre{params[0], params[1], params[2], params[3], params[4], params[5]};
+        return 0;
+    }
+    if (fl->kind == NBX_FLOW_BICKLEY_JET) {
+        NBX_REQUIRE(np >= 12, "bickley_jet needs 12 parameters");
+        for (int k = 0; k < 12; ++k) bj->p[k] = params[k];
+        return 0;
+    }
+    set_error("not an analytic flow");
+    return -2;
+}
+
+template <class R>
+Spline3<R> make_spline(const nbx_flow* fl, double p0) {
+    Spline3<R> s;
+    if constexpr (sizeof(R) == 8) s.C = fl->coef;
+    else s.C = fl->coef32;
+    for (int d = 0; d < 3; ++d) {
+        s.lo[d] = fl->grid9[3 * d];
+        s.hi[d] = fl->grid9[3 * d + 1];
+        const int n = static_cast<int>(fl->grid9[3 * d + 2]);
+        s.dinv[d] = (n - 1) / (s.hi[d] - s.lo[d]);
+        s.nm2[d] = n - 2;
+    }
+    const int nx = static_cast<int>(fl->grid9[5]), ny = static_cast<int>(fl->grid9[8]);
+    s.sx = ny + 2;
+    s.st = (nx + 2) * (ny + 2);
+    s.spherical = fl->spherical;
+    s.extrap = fl->extrap;
+    s.r = fl->r;
+    s.p0 = p0;
+    return s;
+}
+
+}  // namespace nbx
+
+using namespace nbx;
+
+extern "C" {
+
+const char* nbx_last_error(void) { return nbx::g_last_error.c_str(); }
+
+int nbx_version(void) { return 1; }
+
+int nbx_flow_create_analytic(int kind, nbx_flow** out) {
+    NBX_REQUIRE(out != nullptr, "out is NULL");
+    NBX_REQUIRE(kind == NBX_FLOW_DOUBLE_GYRE || kind == NBX_FLOW_BICKLEY_JET,
+                "unknown analytic flow kind");
+    nbx_flow* fl = new nbx_flow();
+    fl->kind = kind;
+    *out = fl;
+    return 0;
+}
+
+int nbx_flow_create_spline3(const double* grid9_host, const double* Cu, const double* Cv,
+                            int spherical, int extrap, double r, nbx_flow** out,
+                            nbx_stream stream) {
+    NBX_REQUIRE(out != nullptr && grid9_host != nullptr && Cu != nullptr && Cv != nullptr,
+                "NULL argument");
+    NBX_REQUIRE(spherical >= 0 && spherical <= 2, "spherical must be 0, 1 or 2");
+    NBX_REQUIRE(extrap >= 0 && extrap <= 2, "unknown extrapolation mode");
+    int n[3];
+    for (int d = 0; d < 3; ++d) {
+        n[d] = static_cast<int>(grid9_host[3 * d + 2]);
+        NBX_REQUIRE(n[d] >= 2 && n[d] == grid9_host[3 * d + 2],
+                    "spline grids need at least 2 points per axis");
+        NBX_REQUIRE(grid9_host[3 * d + 1] > grid9_host[3 * d], "grid axes must be ascending");
+    }
+    const int64_t ncoef = static_cast<int64_t>(n[0] + 2) * (n[1] + 2) * (n[2] + 2);
+    nbx_flow* fl = new nbx_flow();
+    fl->kind = NBX_FLOW_SPLINE3;
+    std::memcpy(fl->grid9, grid9_host, sizeof(fl->grid9));
+    fl->spherical = spherical;
+    fl->extrap = extrap;
+    fl->r = r;
+    fl->ncoef = ncoef;
+    if (hipMalloc(&fl->coef, sizeof(double2) * ncoef) != hipSuccess ||
+        hipMalloc(&fl->coef32, sizeof(float2) * ncoef) != hipSuccess) {
+        nbx_flow_destroy(fl);
+        set_error("hipMalloc of spline coefficients failed");
+        return -1;
+    }
+    hipStream_t s = as_stream(stream);
+    k_interleave<<<blocks_for(ncoef, 256) < 4096 ? blocks_for(ncoef, 256) : 4096, 256, 0, s>>>(
+        Cu, Cv, ncoef, fl->coef, fl->coef32);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        nbx_flow_destroy(fl);
+        set_error(std::string("interleave launch: ") + hipGetErrorString(e));
+        return -1;
+    }
+    *out = fl;
+    return 0;
+}
+
+void nbx_flow_destroy(nbx_flow* fl) {
+    if (!fl) return;
+    if (fl->coef) (void)hipFree(fl->coef);
+    if (fl->coef32) (void)hipFree(fl->coef32);
+    delete fl;
+}
+
+int nbx_prefilter3(const double* U, int nt, int nx, int ny, double* C, nbx_stream stream) {
+    NBX_REQUIRE(U != nullptr && C != nullptr, "NULL argument");
+    NBX_REQUIRE(nt >= 1 && nx >= 1 && ny >= 1, "empty grid");
+    hipStream_t s = as_stream(stream);
+    const long total = static_cast<long>(nt + 2) * (nx + 2) * (ny + 2);
+    const unsigned nb = blocks_for(total, 256) < 8192 ? blocks_for(total, 256) : 8192;
+    k_prefilter_scatter<<<nb, 256, 0, s>>>(U, nt, nx, ny, C);
+    NBX_LAUNCHED("prefilter_scatter");
+    const long lines2 = static_cast<long>(nt) * nx;
+    k_prefilter_axis<<<blocks_for(lines2, 128), 128, 0, s>>>(C, nt, nx, ny, 2);
+    NBX_LAUNCHED("prefilter_axis_y");
+    const long lines1 = static_cast<long>(nt) * (ny + 2);
+    k_prefilter_axis<<<blocks_for(lines1, 128), 128, 0, s>>>(C, nt, nx, ny, 1);
+    NBX_LAUNCHED("prefilter_axis_x");
+    const long lines0 = static_cast<long>(nx + 2) * (ny + 2);
+    k_prefilter_axis<<<blocks_for(lines0, 128), 128, 0, s>>>(C, nt, nx, ny, 0);
+    NBX_LAUNCHED("prefilter_axis_t");
+    return 0;
+}
+
+int nbx_flow_eval(const nbx_flow* fl, const double* params, int np, const double* t,
+                  const double* pts, int64_t n, double* out, nbx_stream stream) {
+    NBX_REQUIRE(fl != nullptr && params != nullptr && np >= 1, "flow and params are required");
+    if (n == 0) return 0;
+    NBX_REQUIRE(t != nullptr && pts != nullptr && out != nullptr, "NULL array");
+    hipStream_t s = as_stream(stream);
+    const auto* P = reinterpret_cast<const double2*>(pts);
+    auto* O = reinterpret_cast<double2*>(out);
+    const unsigned nb = blocks_for(n, 256);
+    if (fl->kind == NBX_FLOW_SPLINE3) {
+        k_flow_eval<<<nb, 256, 0, s>>>(make_spline<double>(fl, params[0]), t, P, n, O);
+    } else {
+        DoubleGyre dg{};
+        BickleyJet bj{};
+        int rc = make_analytic(fl, params, np, &dg, &bj);
+        if (rc) return rc;
+        if (fl->kind == NBX_FLOW_DOUBLE_GYRE) k_flow_eval<<<nb, 256, 0, s>>>(dg, t, P, n, O);
+        else k_flow_eval<<<nb, 256, 0, s>>>(bj, t, P, n, O);
+    }
+    NBX_LAUNCHED("flow_eval");
+    return 0;
+}
+
+int nbx_flowmap_grid_2d(const nbx_flow* fl, double t0, double T, const double* x, int nx,
+                        const double* y, int ny, const double* params, int np, double rtol,
+                        double atol, const uint8_t* mask, int64_t max_steps, int precision,
+                        double* out, int32_t* status, int32_t* nsteps, nbx_stream stream) {
+    NBX_REQUIRE(fl != nullptr, "flow handle is NULL");
+    NBX_REQUIRE(params != nullptr && np >= 1, "params must hold at least the direction");
+    NBX_REQUIRE(nx >= 0 && ny >= 0, "negative grid size");
+    NBX_REQUIRE(rtol > 0 && atol >= 0, "tolerances must be positive");
+    NBX_REQUIRE(max_steps >= 1, "max_steps must be >= 1");
+    NBX_REQUIRE(precision == NBX_FP64, "only the FP64 flow map is available in this entry point");
+    const int64_t P = static_cast<int64_t>(nx) * ny;
+    if (P == 0) return 0;
+    NBX_REQUIRE(x != nullptr && y != nullptr && out != nullptr, "NULL array");
+    hipStream_t s = as_stream(stream);
+    const double s0 = params[0] * t0, s1 = params[0] * (t0 + T);
+    auto* O = reinterpret_cast<double2*>(out);
+    const unsigned nb = blocks_for(P, 256);
+    const long long nmax = max_steps;
+    if (fl->kind == NBX_FLOW_SPLINE3) {
+        k_flowmap_grid<<<nb, 256, 0, s>>>(make_spline<double>(fl, params[0]), x, nx, y, ny, s0,
+                                          s1, rtol, atol, mask, nmax, O, status, nsteps);
+    } else {
+        DoubleGyre dg{};
+        BickleyJet bj{};
+        int rc = make_analytic(fl, params, np, &dg, &bj);
+        if (rc) return rc;
+        if (fl->kind == NBX_FLOW_DOUBLE_GYRE)
+            k_flowmap_grid<<<nb, 256, 0, s>>>(dg, x, nx, y, ny, s0, s1, rtol, atol, mask, nmax, O,
+                                              status, nsteps);
+        else
+            k_flowmap_grid<<<nb, 256, 0, s>>>(bj, x, nx, y, ny, s0, s1, rtol, atol, mask, nmax, O,
+                                              status, nsteps);
+    }
+    NBX_LAUNCHED("flowmap_grid");
+    return 0;
+}
+
+int nbx_flowmap_pts(const nbx_flow* fl, double t0, double T, const double* pts, int64_t npts,
+                    const double* params, int np, double rtol, double atol, const uint8_t* mask,
+                    int64_t max_steps, int precision, double* out, int32_t* status,
+                    int32_t* nsteps, nbx_stream stream) {
+    NBX_REQUIRE(fl != nullptr, "flow handle is NULL");
+    NBX_REQUIRE(params != nullptr && np >= 1, "params must hold at least the direction");
+    NBX_REQUIRE(npts >= 0, "negative point count");
+    NBX_REQUIRE(rtol > 0 && atol >= 0, "tolerances must be positive");
+    NBX_REQUIRE(max_steps >= 1, "max_steps must be >= 1");
+    NBX_REQUIRE(precision == NBX_FP64, "only the FP64 flow map is available in this entry point");
+    if (npts == 0) return 0;
+    NBX_REQUIRE(pts != nullptr && out != nullptr, "NULL array");
+    hipStream_t s = as_stream(stream);
+    const double s0 = params[0] * t0, s1 = params[0] * (t0 + T);
+    const auto* Pp = reinterpret_cast<const double2*>(pts);
+    auto* O = reinterpret_cast<double2*>(out);
+    const unsigned nb = blocks_for(npts, 256);
+    const long long nmax = max_steps;
+    if (fl->kind == NBX_FLOW_SPLINE3) {
+        k_flowmap_pts<<<nb, 256, 0, s>>>(make_spline<double>(fl, params[0]), Pp, npts, s0, s1,
+                                         rtol, atol, mask, nmax, O, status, nsteps);
+    } else {
+        DoubleGyre dg{};
+        BickleyJet bj{};
+        int rc = make_analytic(fl, params, np, &dg, &bj);
+        if (rc) return rc;
+        if (fl->kind == NBX_FLOW_DOUBLE_GYRE)
+            k_flowmap_pts<<<nb, 256, 0, s>>>(dg, Pp, npts, s0, s1, rtol, atol, mask, nmax, O,
+                                             status, nsteps);
+        else
+            k_flowmap_pts<<<nb, 256, 0, s>>>(bj, Pp, npts, s0, s1, rtol, atol, mask, nmax, O,
+                                             status, nsteps);
+    }
+    NBX_LAUNCHED("flowmap_pts");
+    return 0;
+}
+
+}  // extern "C"

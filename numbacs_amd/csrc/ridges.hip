@@ -1,0 +1,751 @@
+// FTLE ridge extraction: percentile threshold, ridge predicate, stream
+// compaction, 8-connected labelling and grouping into ridges.
+//
+// Reference: ftle_ridge_pts (extraction/ridges.py:9-76), _ftle_ridges
+// (ridges.py:79-154) and ftle_ridges (ridges.py:186-229), which use
+// np.percentile (linear method) and scipy.ndimage.label with a full 3x3
+// structure.
+//
+// Device pipeline (no host round trip until the final counts):
+//   percentile  : two radix selects (8 passes of 8 bits over order-preserving
+//                 64-bit keys) for the order statistics around the virtual
+//                 index, then numpy's two-sided lerp -- f_min stays in HBM;
+//   predicate   : one thread per grid point, the reference's central
+//                 differences and Steger sub-pixel test;
+//   compaction  : three-phase exclusive scan over the ridge flags, so points
+//                 come out in raveled (i*ny+j) order;
+//   labelling   : lock-free union-find (atomicMin linking of the larger root
+//                 under the smaller) over the 4 raster-earlier neighbours,
+//                 then path flattening; every root is its component's
+//                 smallest raveled index, so ordering components by root is
+//                 scipy.ndimage.label's raster order of first pixels;
+//   grouping    : bitonic sort of (root << 32 | index) keys, segment
+//                 boundaries, min_ridge_pts filter and a scatter.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace nbx {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 1024 flags per block
+
+constexpr int kSortThreads = 512;
+constexpr int kSortChunk = 2 * kSortThreads;  // keys per LDS chunk
+
+struct SelState {
+    unsigned long long prefix[2];
+    unsigned long long mask[2];
+    long long k[2];
+    unsigned int hist[2][256];
+    unsigned int nan_count;
+    double result;
+};
+
+__device__ __forceinline__ unsigned long long order_key(double v) {
+    const unsigned long long b = __double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double key_value(unsigned long long k) {
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double(static_cast<long long>(b));
+}
+
+__global__ void k_sel_init(SelState* st, long long k_lo, long long k_hi) {
+    const int t = threadIdx.x;
+    if (t < 256) {
+        st->hist[0][t] = 0;
+        st->hist[1][t] = 0;
+    }
+    if (t == 0) {
+        st->prefix[0] = st->prefix[1] = 0;
+        st->mask[0] = st->mask[1] = 0;
+        st->k[0] = k_lo;
+        st->k[1] = k_hi;
+        st->nan_count = 0;
+        st->result = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sel_hist(const double* __restrict__ f, int64_t n,
+                                                  SelState* st, int shift, int count_nan) {
+    __shared__ unsigned int h[2][256];
+    __shared__ unsigned int nans;
+    const int t = threadIdx.x;
+    h[0][t] = 0;
+    h[1][t] = 0;
+    if (t == 0) nans = 0;
+    __syncthreads();
+    const unsigned long long p0 = st->prefix[0], m0 = st->mask[0];
+    const unsigned long long p1 = st->prefix[1], m1 = st->mask[1];
+    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + t; k < n;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double v = f[k];
+        if (count_nan && v != v) atomicAdd(&nans, 1u);
+        const unsigned long long key = order_key(v);
+        const unsigned int bin = static_cast<unsigned int>((key >> shift) & 0xffu);
+        if ((key & m0) == p0) atomicAdd(&h[0][bin], 1u);
+        if ((key & m1) == p1) atomicAdd(&h[1][bin], 1u);
+    }
+    __syncthreads();
+    if (h[0][t]) atomicAdd(&st->hist[0][t], h[0][t]);
+    if (h[1][t]) atomicAdd(&st->hist[1][t], h[1][t]);
+    if (t == 0 && nans) atomicAdd(&st->nan_count, nans);
+}
+
+__global__ void k_sel_pick(SelState* st, int shift) {
+    const int s = threadIdx.x;
+    if (s < 2) {
+        long long k = st->k[s];
+        long long cum = 0;
+        int pick = 255;
+        for (int b = 0; b < 256; ++b) {
+            const long long c = st->hist[s][b];
+            if (cum + c > k) {
+                pick = b;
+                break;
+            }
+            cum += c;
+        }
+        st->k[s] = k - cum;
+        st->prefix[s] |= static_cast<unsigned long long>(pick) << shift;
+        st->mask[s] |= 0xffull << shift;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) {
+        st->hist[0][b] = 0;
+        st->hist[1][b] = 0;
+    }
+}
+
+// numpy's _lerp: a + (b-a)*g, or b - (b-a)*(1-g) when g >= 0.5
+__global__ void k_sel_finish(SelState* st, double gamma, double* out) {
+    const double a = key_value(st->prefix[0]);
+    const double b = key_value(st->prefix[1]);
+    const double diff = b - a;
+    double r = gamma >= 0.5 ? b - diff * (1 - gamma) : a + diff * gamma;
+    if (st->nan_count) r = __longlong_as_double(0x7ff8000000000000ll);
+    st->result = r;
+    if (out) *out = r;
+}
+
+int percentile_device(const double* f, int64_t n, double q, SelState* st, double* out,
+                      hipStream_t s) {
+    NBX_REQUIRE(n >= 1, "percentile of an empty array");
+    NBX_REQUIRE(q >= 0 && q <= 100, "percentile must be in [0, 100]");
+    // numpy 'linear': virtual index (n-1) * (q/100); neighbours clamped to the
+    // last element at or above n-1
+    const double vi = static_cast<double>(n - 1) * (q / 100.0);
+    const double lo = floor(vi);
+    long long klo = static_cast<long long>(lo);
+    long long khi = klo + 1;
+    if (vi >= static_cast<double>(n - 1)) klo = khi = n - 1;
+    if (khi > n - 1) khi = n - 1;
+    const double gamma = vi - lo;
+    k_sel_init<<<1, 256, 0, s>>>(st, klo, khi);
+    NBX_LAUNCHED("sel_init");
+    unsigned nb = blocks_for(n, 256);
+    if (nb > 1024) nb = 1024;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
+        k_sel_hist<<<nb, 256, 0, s>>>(f, n, st, shift, pass == 0);
+        NBX_LAUNCHED("sel_hist");
+        k_sel_pick<<<1, 256, 0, s>>>(st, shift);
+        NBX_LAUNCHED("sel_pick");
+    }
+    k_sel_finish<<<1, 1, 0, s>>>(st, gamma, out);
+    NBX_LAUNCHED("sel_finish");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Ridge predicate (ridges.py:52-74): flag[k] = 1 and pts[k] = sub-pixel ridge
+// point where the test passes, flag[k] = 0 elsewhere.
+__global__ __launch_bounds__(256) void k_ridge_predicate(
+    const double* __restrict__ f, const double2* __restrict__ evec, const double* __restrict__ xs,
+    const double* __restrict__ ys, int nx, int ny, double dx, double dy, double sdd_thresh,
+    const double* __restrict__ fmin_dev, int32_t* __restrict__ flag, double2* __restrict__ pts) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const long k = static_cast<long>(i) * ny + j;
+    int32_t hit = 0;
+    if (i >= 2 && i < nx - 2 && j >= 2 && j < ny - 2) {
+        const double f_min = fmin_dev ? *fmin_dev : 0.0;
+        const double f0 = f[k];
+        if (f0 > f_min) {
+            const double fe = f[k + ny], fw = f[k - ny], fn = f[k + 1], fs = f[k - 1];
+            const double fx = (fe - fw) / (2 * dx);
+            const double fy = (fn - fs) / (2 * dy);
+            const double fxx = (fe - 2 * f0 + fw) / (dx * dx);
+            const double fyy = (fn - 2 * f0 + fs) / (dy * dy);
+            const double fxy = (f[k + ny + 1] - f[k + ny - 1] - f[k - ny + 1] + f[k - ny - 1]) /
+                               (4 * dx * dy);
+            const double2 e = evec[k];
+            const double ex = e.x, ey = e.y;
+            const double c2 = ex * (fxx * ex + fxy * ey) + ey * (fxy * ex + fyy * ey);
+            if (c2 < -sdd_thresh) {
+                const double t = -(fx * ex + fy * ey) / c2;
+                if (fabs(t * ex) <= dx / 2 && fabs(t * ey) <= dy / 2) {
+                    pts[k] = make_double2(xs[i] + t * ex, ys[j] + t * ey);
+                    hit = 1;
+                }
+            }
+        }
+    }
+    flag[k] = hit;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of int32 values (sums must fit in int32).
+
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* sh, int32_t* total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int32_t add = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const int32_t incl = sh[t];
+    *total = sh[kScanThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const int32_t* __restrict__ in,
+                                                             int64_t n, int32_t* __restrict__ out,
+                                                             int32_t* __restrict__ tile_sums) {
+    __shared__ int32_t sh[kScanThreads];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        v[q] = base + q < n ? in[base + q] : 0;
+        sum += v[q];
+    }
+    int32_t total;
+    int32_t run = block_exclusive_scan(sum, sh, &total);
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        if (base + q < n) out[base + q] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of the tile sums in place; grand total to *total
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(int32_t* __restrict__ sums,
+                                                            int64_t m, int32_t* __restrict__ total) {
+    __shared__ int32_t sh[kScanThreads];
+    int32_t carry = 0;
+    for (int64_t base = 0; base < m; base += kScanThreads) {
+        const int64_t k = base + threadIdx.x;
+        const int32_t v = k < m ? sums[k] : 0;
+        int32_t chunk;
+        const int32_t ex = block_exclusive_scan(v, sh, &chunk);
+        if (k < m) sums[k] = carry + ex;
+        carry += chunk;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int32_t* __restrict__ out, int64_t n,
+                                                  const int32_t* __restrict__ tile_sums) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k < n) out[k] += tile_sums[k / kScanTile];
+}
+
+__global__ void k_set_zero(int32_t* p) { *p = 0; }
+
+// out = exclusive scan of in; *total (device) = sum.  tmp holds
+// ceil(n / 1024) ints.
+int exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* tmp, int32_t* total,
+                   hipStream_t s) {
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (n == 0) {
+        k_set_zero<<<1, 1, 0, s>>>(total);
+        NBX_LAUNCHED("set_zero");
+        return 0;
+    }
+    k_scan_tiles<<<static_cast<unsigned>(tiles), kScanThreads, 0, s>>>(in, n, out, tmp);
+    NBX_LAUNCHED("scan_tiles");
+    k_scan_sums<<<1, kScanThreads, 0, s>>>(tmp, tiles, total);
+    NBX_LAUNCHED("scan_sums");
+    k_scan_add<<<blocks_for(n, 256), 256, 0, s>>>(out, n, tmp);
+    NBX_LAUNCHED("scan_add");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Compaction of flagged points in raveled order.
+__global__ void k_compact(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                          const double2* __restrict__ pts_dense, int64_t n,
+                          double2* __restrict__ pts_out, int32_t* __restrict__ idx_out) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k >= n || !flag[k]) return;
+    const int32_t p = pos[k];
+    if (pts_out) pts_out[p] = pts_dense[k];
+    if (idx_out) idx_out[p] = static_cast<int32_t>(k);
+}
+
+// ---------------------------------------------------------------------------
+// Union-find labelling.
+__device__ __forceinline__ int32_t uf_load(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int32_t uf_root(const int32_t* par, int32_t a) {
+    int32_t p = uf_load(par + a);
+    while (p != a) {
+        a = p;
+        p = uf_load(par + a);
+    }
+    return a;
+}
+
+__device__ void uf_unite(int32_t* par, int32_t a, int32_t b) {
+    for (;;) {
+        a = uf_root(par, a);
+        b = uf_root(par, b);
+        if (a == b) return;
+        if (a < b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        // a > b: hang root a under b; if a stopped being a root meanwhile,
+        // continue with what it now points to
+        const int32_t old = atomicMin(par + a, b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+__global__ void k_uf_init(const int32_t* __restrict__ flag, int64_t n, int32_t* __restrict__ par) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k < n) par[k] = flag[k] ? static_cast<int32_t>(k) : -1;
+}
+
+__global__ __launch_bounds__(256) void k_uf_union(const int32_t* __restrict__ flag, int nx, int ny,
+                                                  int32_t* par) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const int32_t k = i * ny + j;
+    if (!flag[k]) return;
+    if (j > 0 && flag[k - 1]) uf_unite(par, k, k - 1);
+    if (i > 0) {
+        const int32_t up = k - ny;
+        if (j > 0 && flag[up - 1]) uf_unite(par, k, up - 1);
+        if (flag[up]) uf_unite(par, k, up);
+        if (j + 1 < ny && flag[up + 1]) uf_unite(par, k, up + 1);
+    }
+}
+
+__global__ void k_uf_flatten(const int32_t* __restrict__ flag, int64_t n, int32_t* par) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k < n && flag[k]) par[k] = uf_root(par, static_cast<int32_t>(k));
+}
+
+__global__ void k_bytes_to_flags(const uint8_t* __restrict__ b, int64_t n,
+                                 int32_t* __restrict__ flag) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k < n) flag[k] = b[k] ? 1 : 0;
+}
+
+__global__ void k_is_root(const int32_t* __restrict__ par, int64_t n, int32_t* __restrict__ out) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k < n) out[k] = par[k] == static_cast<int32_t>(k) ? 1 : 0;
+}
+
+__global__ void k_label_out(const int32_t* __restrict__ par, const int32_t* __restrict__ root_rank,
+                            int64_t n, int32_t* __restrict__ labels) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k < n) labels[k] = par[k] >= 0 ? root_rank[par[k]] + 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Bitonic sort of 64-bit keys; N is a power of two and a multiple of
+// kSortChunk.
+
+__device__ __forceinline__ void cmp_swap(unsigned long long& a, unsigned long long& b, bool up) {
+    if ((a > b) == up) {
+        const unsigned long long t = a;
+        a = b;
+        b = t;
+    }
+}
+
+// all stages k <= kSortChunk inside one chunk
+__global__ __launch_bounds__(kSortThreads) void k_bitonic_chunk_full(unsigned long long* keys) {
+    __shared__ unsigned long long sh[kSortChunk];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortChunk;
+    const int t = threadIdx.x;
+    sh[t] = keys[base + t];
+    sh[t + kSortThreads] = keys[base + t + kSortThreads];
+    __syncthreads();
+    for (int k = 2; k <= kSortChunk; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int lo = 2 * j * (t / j) + (t % j);
+            const bool up = (((base + lo) & k) == 0);
+            cmp_swap(sh[lo], sh[lo + j], up);
+            __syncthreads();
+        }
+    }
+    keys[base + t] = sh[t];
+    keys[base + t + kSortThreads] = sh[t + kSortThreads];
+}
+
+// stage k (> kSortChunk), steps j < kSortChunk inside one chunk
+__global__ __launch_bounds__(kSortThreads) void k_bitonic_chunk_tail(unsigned long long* keys,
+                                                                     int64_t k) {
+    __shared__ unsigned long long sh[kSortChunk];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortChunk;
+    const int t = threadIdx.x;
+    sh[t] = keys[base + t];
+    sh[t + kSortThreads] = keys[base + t + kSortThreads];
+    __syncthreads();
+    const bool up = ((base & k) == 0);
+    for (int j = kSortChunk >> 1; j > 0; j >>= 1) {
+        const int lo = 2 * j * (t / j) + (t % j);
+        cmp_swap(sh[lo], sh[lo + j], up);
+        __syncthreads();
+    }
+    keys[base + t] = sh[t];
+    keys[base + t + kSortThreads] = sh[t + kSortThreads];
+}
+
+__global__ void k_bitonic_global(unsigned long long* keys, int64_t N, int64_t k, int64_t j) {
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= N / 2) return;
+    const int64_t lo = 2 * j * (t / j) + (t % j);
+    const bool up = ((lo & k) == 0);
+    unsigned long long a = keys[lo], b = keys[lo + j];
+    if ((a > b) == up) {
+        keys[lo] = b;
+        keys[lo + j] = a;
+    }
+}
+
+int bitonic_sort(unsigned long long* keys, int64_t N, hipStream_t s) {
+    const unsigned chunks = static_cast<unsigned>(N / kSortChunk);
+    k_bitonic_chunk_full<<<chunks, kSortThreads, 0, s>>>(keys);
+    NBX_LAUNCHED("bitonic_chunk_full");
+    for (int64_t k = 2 * kSortChunk; k <= N; k <<= 1) {
+        for (int64_t j = k >> 1; j >= kSortChunk; j >>= 1) {
+            k_bitonic_global<<<blocks_for(N / 2, 256), 256, 0, s>>>(keys, N, k, j);
+            NBX_LAUNCHED("bitonic_global");
+        }
+        k_bitonic_chunk_tail<<<chunks, kSortThreads, 0, s>>>(keys, k);
+        NBX_LAUNCHED("bitonic_chunk_tail");
+    }
+    return 0;
+}
+
+__global__ void k_make_keys(const int32_t* __restrict__ idx, const int32_t* __restrict__ par,
+                            const int32_t* __restrict__ count_dev, int64_t N,
+                            unsigned long long* __restrict__ keys) {
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= N) return;
+    const int32_t cnt = *count_dev;
+    if (t < cnt) {
+        const int32_t k = idx[t];
+        keys[t] = (static_cast<unsigned long long>(static_cast<uint32_t>(par[k])) << 32) |
+                  static_cast<uint32_t>(k);
+    } else {
+        keys[t] = ~0ull;
+    }
+}
+
+// seg_start[t] = 1 where a new component begins in the sorted keys
+__global__ void k_seg_start(const unsigned long long* __restrict__ keys,
+                            const int32_t* __restrict__ count_dev, int64_t cap,
+                            int32_t* __restrict__ seg_start) {
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= cap) return;
+    const int32_t cnt = *count_dev;
+    int32_t v = 0;
+    if (t < cnt) v = (t == 0 || (keys[t] >> 32) != (keys[t - 1] >> 32)) ? 1 : 0;
+    seg_start[t] = v;
+}
+
+// seg_first[s] = position of the first key of segment s; seg_first[nseg] = cnt
+__global__ void k_seg_first(const int32_t* __restrict__ seg_start,
+                            const int32_t* __restrict__ seg_ex, const int32_t* __restrict__ count_dev,
+                            const int32_t* __restrict__ nseg_dev, int64_t cap,
+                            int32_t* __restrict__ seg_first) {
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t < cap && t < *count_dev && seg_start[t]) seg_first[seg_ex[t]] = static_cast<int32_t>(t);
+    if (t == 0) seg_first[*nseg_dev] = *count_dev;
+}
+
+// kept ridge sizes: len or 0
+__global__ void k_seg_keep(const int32_t* __restrict__ seg_first, const int32_t* __restrict__ nseg_dev,
+                           int64_t cap, int min_pts, int32_t* __restrict__ kept_len,
+                           int32_t* __restrict__ kept_flag) {
+    const int64_t s = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (s >= cap) return;
+    int32_t len = 0;
+    if (s < *nseg_dev) len = seg_first[s + 1] - seg_first[s];
+    const bool keep = len >= min_pts && len > 0;
+    kept_len[s] = keep ? len : 0;
+    kept_flag[s] = keep ? 1 : 0;
+}
+
+__global__ void k_group_scatter(const unsigned long long* __restrict__ keys,
+                                const int32_t* __restrict__ seg_start,
+                                const int32_t* __restrict__ seg_ex,
+                                const int32_t* __restrict__ seg_first,
+                                const int32_t* __restrict__ kept_flag,
+                                const int32_t* __restrict__ out_off,
+                                const int32_t* __restrict__ ridge_idx,
+                                const double2* __restrict__ pts_dense,
+                                const int32_t* __restrict__ count_dev, int64_t cap,
+                                double2* __restrict__ pts_out, int64_t* __restrict__ offsets) {
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= cap || t >= *count_dev) return;
+    // segment of t: inclusive count of starts up to t, minus one
+    const int32_t s = seg_ex[t] + seg_start[t] - 1;
+    if (!kept_flag[s]) return;
+    const int32_t k = static_cast<int32_t>(keys[t] & 0xffffffffull);
+    const int32_t r = out_off[s] + (static_cast<int32_t>(t) - seg_first[s]);
+    pts_out[r] = pts_dense[k];
+    if (t == seg_first[s]) offsets[ridge_idx[s]] = out_off[s];
+}
+
+__global__ void k_offsets_end(const int32_t* __restrict__ nridges_dev,
+                              const int32_t* __restrict__ total_dev, int64_t* __restrict__ offsets) {
+    offsets[*nridges_dev] = *total_dev;
+}
+
+// Bump allocator over one workspace block.
+struct Arena {
+    char* base;
+    size_t used;
+    template <class T>
+    T* take(int64_t count) {
+        const size_t bytes = (sizeof(T) * static_cast<size_t>(count) + 255) & ~static_cast<size_t>(255);
+        T* p = reinterpret_cast<T*>(base + used);
+        used += bytes;
+        return p;
+    }
+};
+
+size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+// Ridge predicate + optional percentile; leaves flag/pts_dense in the arena.
+int run_predicate(const double* f, const double* evec, const double* x, const double* y, int nx,
+                  int ny, double dx, double dy, double sdd_thresh, double percentile,
+                  SelState* st, int32_t* flag, double2* pts_dense, hipStream_t s) {
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    const double* fmin_dev = nullptr;
+    if (percentile != 0) {
+        int rc = percentile_device(f, n, percentile, st, nullptr, s);
+        if (rc) return rc;
+        fmin_dev = &st->result;
+    }
+    const dim3 block(64, 4), grid((ny + 63) / 64, (nx + 3) / 4);
+    k_ridge_predicate<<<grid, block, 0, s>>>(f, reinterpret_cast<const double2*>(evec), x, y, nx,
+                                             ny, dx, dy, sdd_thresh, fmin_dev, flag, pts_dense);
+    NBX_LAUNCHED("ridge_predicate");
+    return 0;
+}
+
+int check_grid(int nx, int ny) {
+    NBX_REQUIRE(nx >= 0 && ny >= 0, "negative grid size");
+    NBX_REQUIRE(static_cast<int64_t>(nx) * ny < (static_cast<int64_t>(1) << 31) - 1,
+                "grid too large for 32-bit point indices");
+    return 0;
+}
+
+}  // namespace
+}  // namespace nbx
+
+using namespace nbx;
+
+extern "C" {
+
+int nbx_percentile(const double* f, int64_t n, double q, double* result, nbx_stream stream) {
+    NBX_REQUIRE(f != nullptr && result != nullptr, "NULL array");
+    int err = 0;
+    auto* st = static_cast<SelState*>(workspace(2, sizeof(SelState), &err));
+    if (err) return err;
+    return percentile_device(f, n, q, st, result, as_stream(stream));
+}
+
+int nbx_ridge_pts(const double* f, const double* evec, const double* x, int nx, const double* y,
+                  int ny, double dx, double dy, double sdd_thresh, double percentile, double* pts,
+                  int64_t* npts_host, nbx_stream stream) {
+    if (int rc = check_grid(nx, ny)) return rc;
+    NBX_REQUIRE(npts_host != nullptr, "npts_host is NULL");
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    *npts_host = 0;
+    if (n == 0) return 0;
+    NBX_REQUIRE(f && evec && x && y && pts, "NULL array");
+    hipStream_t s = as_stream(stream);
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    const size_t bytes = align256(sizeof(SelState)) + align256(4 * n) * 2 + align256(16 * n) +
+                         align256(4 * (tiles + 1)) + align256(4);
+    int err = 0;
+    Arena A{static_cast<char*>(workspace(1, bytes, &err)), 0};
+    if (err) return err;
+    auto* st = A.take<SelState>(1);
+    auto* flag = A.take<int32_t>(n);
+    auto* pos = A.take<int32_t>(n);
+    auto* dense = A.take<double2>(n);
+    auto* tmp = A.take<int32_t>(tiles + 1);
+    auto* total = A.take<int32_t>(1);
+    if (int rc = run_predicate(f, evec, x, y, nx, ny, dx, dy, sdd_thresh, percentile, st, flag,
+                               dense, s))
+        return rc;
+    if (int rc = exclusive_scan(flag, n, pos, tmp, total, s)) return rc;
+    k_compact<<<blocks_for(n, 256), 256, 0, s>>>(flag, pos, dense, n,
+                                                 reinterpret_cast<double2*>(pts), nullptr);
+    NBX_LAUNCHED("compact");
+    int32_t cnt = 0;
+    NBX_HIP(hipMemcpyAsync(&cnt, total, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    *npts_host = cnt;
+    return 0;
+}
+
+int nbx_ridges(const double* f, const double* evec, const double* x, int nx, const double* y,
+               int ny, double dx, double dy, double sdd_thresh, double percentile,
+               int min_ridge_pts, double* pts, int64_t* offsets, int64_t* nridges_host,
+               int64_t* npts_host, nbx_stream stream) {
+    if (int rc = check_grid(nx, ny)) return rc;
+    NBX_REQUIRE(nridges_host != nullptr && npts_host != nullptr, "NULL count pointer");
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    *nridges_host = 0;
+    *npts_host = 0;
+    if (n == 0) return 0;
+    NBX_REQUIRE(f && evec && x && y && pts && offsets, "NULL array");
+    hipStream_t s = as_stream(stream);
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    int64_t N = kSortChunk;
+    while (N < n) N <<= 1;
+    const size_t bytes = align256(sizeof(SelState)) + align256(4 * n) * 9 + align256(16 * n) +
+                         align256(4 * (tiles + 1)) + align256(4) * 6 + align256(8 * N) +
+                         align256(4 * (n + 1));
+    int err = 0;
+    Arena A{static_cast<char*>(workspace(1, bytes, &err)), 0};
+    if (err) return err;
+    auto* st = A.take<SelState>(1);
+    auto* flag = A.take<int32_t>(n);
+    auto* pos = A.take<int32_t>(n);
+    auto* dense = A.take<double2>(n);
+    auto* tmp = A.take<int32_t>(tiles + 1);
+    auto* count = A.take<int32_t>(1);
+    auto* par = A.take<int32_t>(n);
+    auto* idx = A.take<int32_t>(n);
+    auto* keys = A.take<unsigned long long>(N);
+    auto* seg_start = A.take<int32_t>(n);
+    auto* seg_ex = A.take<int32_t>(n);
+    auto* seg_first = A.take<int32_t>(n + 1);
+    auto* kept_len = A.take<int32_t>(n);
+    auto* kept_flag = A.take<int32_t>(n);
+    auto* out_off = A.take<int32_t>(n);
+    auto* ridge_idx = A.take<int32_t>(n);
+    auto* nseg = A.take<int32_t>(1);
+    auto* total = A.take<int32_t>(1);
+    auto* nridges = A.take<int32_t>(1);
+
+    if (int rc = run_predicate(f, evec, x, y, nx, ny, dx, dy, sdd_thresh, percentile, st, flag,
+                               dense, s))
+        return rc;
+    // label components
+    const unsigned nb = blocks_for(n, 256);
+    k_uf_init<<<nb, 256, 0, s>>>(flag, n, par);
+    NBX_LAUNCHED("uf_init");
+    const dim3 block(64, 4), grid((ny + 63) / 64, (nx + 3) / 4);
+    k_uf_union<<<grid, block, 0, s>>>(flag, nx, ny, par);
+    NBX_LAUNCHED("uf_union");
+    k_uf_flatten<<<nb, 256, 0, s>>>(flag, n, par);
+    NBX_LAUNCHED("uf_flatten");
+    // compact ridge points (raveled order)
+    if (int rc = exclusive_scan(flag, n, pos, tmp, count, s)) return rc;
+    k_compact<<<nb, 256, 0, s>>>(flag, pos, dense, n, nullptr, idx);
+    NBX_LAUNCHED("compact");
+    // order by (component root, raveled index)
+    k_make_keys<<<blocks_for(N, 256), 256, 0, s>>>(idx, par, count, N, keys);
+    NBX_LAUNCHED("make_keys");
+    if (int rc = bitonic_sort(keys, N, s)) return rc;
+    // segments = components
+    k_seg_start<<<nb, 256, 0, s>>>(keys, count, n, seg_start);
+    NBX_LAUNCHED("seg_start");
+    if (int rc = exclusive_scan(seg_start, n, seg_ex, tmp, nseg, s)) return rc;
+    k_seg_first<<<nb, 256, 0, s>>>(seg_start, seg_ex, count, nseg, n, seg_first);
+    NBX_LAUNCHED("seg_first");
+    k_seg_keep<<<nb, 256, 0, s>>>(seg_first, nseg, n, min_ridge_pts, kept_len, kept_flag);
+    NBX_LAUNCHED("seg_keep");
+    if (int rc = exclusive_scan(kept_len, n, out_off, tmp, total, s)) return rc;
+    if (int rc = exclusive_scan(kept_flag, n, ridge_idx, tmp, nridges, s)) return rc;
+    k_group_scatter<<<nb, 256, 0, s>>>(keys, seg_start, seg_ex, seg_first, kept_flag, out_off,
+                                       ridge_idx, dense, count, n,
+                                       reinterpret_cast<double2*>(pts), offsets);
+    NBX_LAUNCHED("group_scatter");
+    k_offsets_end<<<1, 1, 0, s>>>(nridges, total, offsets);
+    NBX_LAUNCHED("offsets_end");
+    int32_t h[2] = {0, 0};
+    NBX_HIP(hipMemcpyAsync(&h[0], nridges, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipMemcpyAsync(&h[1], total, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    *nridges_host = h[0];
+    *npts_host = h[1];
+    return 0;
+}
+
+int nbx_label8(const uint8_t* b, int nx, int ny, int32_t* labels, int32_t* nlabels_host,
+               nbx_stream stream) {
+    if (int rc = check_grid(nx, ny)) return rc;
+    NBX_REQUIRE(nlabels_host != nullptr, "nlabels_host is NULL");
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    *nlabels_host = 0;
+    if (n == 0) return 0;
+    NBX_REQUIRE(b != nullptr && labels != nullptr, "NULL array");
+    hipStream_t s = as_stream(stream);
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    const size_t bytes = align256(4 * n) * 4 + align256(4 * (tiles + 1)) + align256(4);
+    int err = 0;
+    Arena A{static_cast<char*>(workspace(1, bytes, &err)), 0};
+    if (err) return err;
+    auto* flag = A.take<int32_t>(n);
+    auto* par = A.take<int32_t>(n);
+    auto* isroot = A.take<int32_t>(n);
+    auto* rank = A.take<int32_t>(n);
+    auto* tmp = A.take<int32_t>(tiles + 1);
+    auto* total = A.take<int32_t>(1);
+    const unsigned nb = blocks_for(n, 256);
+    k_bytes_to_flags<<<nb, 256, 0, s>>>(b, n, flag);
+    NBX_LAUNCHED("bytes_to_flags");
+    k_uf_init<<<nb, 256, 0, s>>>(flag, n, par);
+    NBX_LAUNCHED("uf_init");
+    const dim3 block(64, 4), grid((ny + 63) / 64, (nx + 3) / 4);
+    k_uf_union<<<grid, block, 0, s>>>(flag, nx, ny, par);
+    NBX_LAUNCHED("uf_union");
+    k_uf_flatten<<<nb, 256, 0, s>>>(flag, n, par);
+    NBX_LAUNCHED("uf_flatten");
+    k_is_root<<<nb, 256, 0, s>>>(par, n, isroot);
+    NBX_LAUNCHED("is_root");
+    if (int rc = exclusive_scan(isroot, n, rank, tmp, total, s)) return rc;
+    k_label_out<<<nb, 256, 0, s>>>(par, rank, n, labels);
+    NBX_LAUNCHED("label_out");
+    int32_t cnt = 0;
+    NBX_HIP(hipMemcpyAsync(&cnt, total, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    *nlabels_host = cnt;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""FTLE ridge extraction (mirrors numbacs.extraction.ridges).
+
+Reference: ftle_ridge_pts (ridges.py:9-76) and ftle_ridges
+(ridges.py:186-229).  The percentile threshold, ridge predicate, compaction,
+8-connected labelling and grouping all run on the GPU; the host only splits
+the grouped point buffer into the list the reference returns.
+"""
+import ctypes
+
+import numpy as np
+
+from .. import _lib
+
+
+def _inputs(f, eigvec_max, x, y):
+    f = np.asarray(f, dtype=np.float64) if not _is_tensor(f) else f
+    nx, ny = int(f.shape[0]), int(f.shape[1])
+    if tuple(eigvec_max.shape) != (nx, ny, 2):
+        raise ValueError("eigvec_max must have shape (nx, ny, 2)")
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    y = np.ascontiguousarray(y, dtype=np.float64).ravel()
+    if len(x) != nx or len(y) != ny:
+        raise ValueError("x and y must match the grid of f")
+    # the reference's spacing (ridges.py:45-46)
+    dx = x[1] - x[0]
+    dy = y[1] - y[0]
+    return (_lib.to_dev(f), _lib.to_dev(eigvec_max), _lib.to_dev(x), _lib.to_dev(y), nx, ny,
+            float(dx), float(dy))
+
+
+def _is_tensor(a):
+    try:
+        import torch
+
+        return isinstance(a, torch.Tensor)
+    except ImportError:
+        return False
+
+
+def ftle_ridge_pts_device(fd, evd, xd, yd, nx, ny, dx, dy, sdd_thresh=0.0, percentile=0):
+    """Device form: returns (pts tensor (capacity nx*ny, 2), count)."""
+    pts = _lib.empty((nx * ny, 2))
+    n = ctypes.c_int64()
+    _lib.check(_lib.load().nbx_ridge_pts(_lib.ptr(fd), _lib.ptr(evd), _lib.ptr(xd), nx,
+                                         _lib.ptr(yd), ny, dx, dy, float(sdd_thresh),
+                                         float(percentile), _lib.ptr(pts), ctypes.byref(n),
+                                         _lib.stream_ptr()))
+    return pts, n.value
+
+
+def ftle_ridges_device(fd, evd, xd, yd, nx, ny, dx, dy, sdd_thresh=0.0, percentile=0,
+                       min_ridge_pts=3):
+    """Device form: returns (pts tensor, offsets tensor, nridges, npts)."""
+    pts = _lib.empty((max(nx * ny, 1), 2))
+    offsets = _lib.empty((nx * ny + 1,), "i64")
+    nr = ctypes.c_int64()
+    npts = ctypes.c_int64()
+    _lib.check(_lib.load().nbx_ridges(_lib.ptr(fd), _lib.ptr(evd), _lib.ptr(xd), nx, _lib.ptr(yd),
+                                      ny, dx, dy, float(sdd_thresh), float(percentile),
+                                      int(min_ridge_pts), _lib.ptr(pts), _lib.ptr(offsets),
+                                      ctypes.byref(nr), ctypes.byref(npts), _lib.stream_ptr()))
+    return pts, offsets, nr.value, npts.value
+
+
+def ftle_ridge_pts(f, eigvec_max, x, y, sdd_thresh=0.0, percentile=0):
+    """Sub-pixel FTLE ridge points (k, 2) in raveled grid order: points with
+    f > percentile(f) (0 when percentile == 0) where the derivative of f
+    along eigvec_max vanishes within the cell and the second directional
+    derivative is below -sdd_thresh."""
+    fd, evd, xd, yd, nx, ny, dx, dy = _inputs(f, eigvec_max, x, y)
+    if nx * ny == 0:
+        return np.zeros((0, 2))
+    pts, n = ftle_ridge_pts_device(fd, evd, xd, yd, nx, ny, dx, dy, sdd_thresh, percentile)
+    return _lib.to_host(pts[:n])
+
+
+def ftle_ridges(f, eigvec_max, x, y, sdd_thresh=0.0, percentile=0, min_ridge_pts=3):
+    """Ridge points grouped into 8-connected ridges: a list of (m, 2) arrays
+    in label order (raster order of each ridge's first point), points in
+    raveled order, ridges with fewer than min_ridge_pts points dropped."""
+    fd, evd, xd, yd, nx, ny, dx, dy = _inputs(f, eigvec_max, x, y)
+    if nx * ny == 0:
+        return []
+    pts, offsets, nr, npts = ftle_ridges_device(fd, evd, xd, yd, nx, ny, dx, dy, sdd_thresh,
+                                                percentile, min_ridge_pts)
+    P = _lib.to_host(pts[:npts])
+    off = _lib.to_host(offsets[:nr + 1])
+    return [P[off[k]:off[k + 1]] for k in range(nr)]
+
+
+def label8(b):
+    """8-connected labels of a boolean grid (scipy.ndimage.label with a full
+    3x3 structure): (labels (nx, ny) int32, count)."""
+    b = np.asarray(b)
+    nx, ny = b.shape
+    bd = _lib.to_dev(b.astype(np.uint8), np.uint8)
+    lab = _lib.empty((nx, ny), "i32")
+    n = ctypes.c_int32()
+    _lib.check(_lib.load().nbx_label8(_lib.ptr(bd), nx, ny, _lib.ptr(lab), ctypes.byref(n),
+                                      _lib.stream_ptr()))
+    return _lib.to_host(lab), n.value

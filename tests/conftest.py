@@ -1,0 +1,92 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+# the reference's fixture world (tests/conftest.py:9-22 of the reference)
+NX, NY = 21, 11
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (runs the HIP library)")
+
+
+def golden(name):
+    return os.path.join(GOLDEN, name)
+
+
+@pytest.fixture
+def coords_dg():
+    return np.linspace(0, 2, NX), np.linspace(0, 1, NY)
+
+
+@pytest.fixture
+def mask_dg():
+    mask = np.zeros((NX, NY), np.bool_)
+    mask[::2, ::2] = True
+    return mask
+
+
+@pytest.fixture
+def fm_data():
+    return np.load(golden("fm.npy"))
+
+
+@pytest.fixture
+def ftle_data():
+    return np.load(golden("ftle.npy"))
+
+
+@pytest.fixture
+def C_eig_data():
+    return np.load(golden("Cevals.npy")), np.load(golden("Cevecs.npy"))
+
+
+@pytest.fixture
+def ridge_pts_data():
+    return np.load(golden("ridge_pts.npy"))
+
+
+@pytest.fixture
+def literals():
+    return np.load(golden("reference_literals.npz"))
+
+
+@pytest.fixture
+def scipy_golden():
+    return np.load(golden("scipy_dop853.npz"))
+
+
+def apply_mask(arr, mask):
+    out = arr.copy()
+    out[mask] = 0.0
+    return out
+
+
+def reconstruct(eigvals, eigvecs):
+    """V diag(l) V^T per grid point (the reference's check,
+    tests/test_diagnostics.py:18-47)."""
+    D = np.zeros_like(eigvecs)
+    D[..., 0, 0] = eigvals[..., 0]
+    D[..., 1, 1] = eigvals[..., 1]
+    return eigvecs @ D @ np.swapaxes(eigvecs, -1, -2)
+
+
+def rounding_spread(run, x, y):
+    """Per-particle response (max over components) of a flow-map function
+    run(x, y) -> (nx, ny, 2) to a few-ulp shift of the initial condition in
+    x and in y: the floor below which two correct implementations with
+    different rounding cannot be told apart on chaotic trajectories."""
+    base = run(x, y)
+    dx = 1e-15 * max(np.abs(x).max(), 1.0)
+    dy = 1e-15 * max(np.abs(y).max(), 1.0)
+    s1 = np.abs(run(x + dx, y) - base).max(-1)
+    s2 = np.abs(run(x, y + dy) - base).max(-1)
+    return base, np.maximum(s1, s2)

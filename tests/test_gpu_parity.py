@@ -1,0 +1,441 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the
+reference's own fixtures.  Tolerances: flow maps 1e-7 relative unless the
+trajectory's own conditioning (measured by a 1-ulp perturbation of the
+initial condition in the oracle) is looser; FTLE 1e-6 relative; integer and
+index results (ridge flags, labels, orderings) exact; percentile bit-exact.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import oracle
+from conftest import apply_mask, reconstruct, rounding_spread
+
+pytestmark = pytest.mark.gpu
+
+T_FIX = 8.0
+
+
+def relerr(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def nb():
+    import numbacs_amd
+    from numbacs_amd import _lib
+
+    _lib.torch()
+    return numbacs_amd
+
+
+# ---------------------------------------------------------------- flows --
+
+
+def test_predefined_velocity_literals(nb, literals):
+    """Velocity of the double gyre and Bickley jet at the 27 points of the
+    reference test (tests/test_flows.py:307-357)."""
+    from numbacs_amd.flows import flow_velocity, get_predefined_flow
+
+    ti = np.array([0.05, 0.12, 0.18])
+    xi = np.array([0.1, 0.4, 0.7])
+    Ti, Xi, Yi = np.meshgrid(ti, xi, xi, indexing="ij")
+    pts = np.column_stack((Xi.ravel(), Yi.ravel()))
+    f, p, dom = get_predefined_flow("double_gyre")
+    v = flow_velocity(f, Ti.ravel(), pts, p)
+    assert np.allclose(v.T, literals["dg_vel"])
+    assert np.allclose(dom, literals["dg_domain"])
+    f, p, _ = get_predefined_flow("bickley_jet")
+    v = flow_velocity(f, Ti.ravel(), pts, p)
+    assert np.allclose(v.T, literals["bickley_vel"])
+    # against the oracle to machine precision
+    assert relerr(v, oracle.rhs_eval("bickley_jet", p, Ti.ravel(), pts)) < 1e-14
+
+
+def test_prefilter_literals(nb, literals):
+    """get_interp_arrays_2D coefficients (tests/test_flows.py:68-126)."""
+    from numbacs_amd.flows import get_interp_arrays_2D
+
+    t = np.array([0.0, 0.1, 0.2])
+    x = np.array([0.0, 0.5, 1.0])
+    T_, X, Y = np.meshgrid(t, x, x, indexing="ij")
+    u = np.sin(X) * np.cos(Y) + np.sin(T_)
+    v = np.cos(X) * np.sin(Y) + np.cos(T_)
+    grid, Cu, Cv = get_interp_arrays_2D(t, x, x, u, v)
+    assert np.allclose(grid, ((0.0, 0.2, 3), (0.0, 1.0, 3), (0.0, 1.0, 3)))
+    assert np.allclose(Cu, literals["prefilter3_Cu"])
+    assert np.allclose(Cv, literals["prefilter3_Cv"])
+
+
+def test_prefilter_vs_oracle_long_axes(nb):
+    from numbacs_amd.flows import get_interp_arrays_2D
+
+    rng = np.random.default_rng(3)
+    U = rng.standard_normal((70, 45, 33))
+    V = rng.standard_normal((70, 45, 33))
+    t, x, y = np.arange(70.0), np.linspace(0, 1, 45), np.linspace(0, 2, 33)
+    _, Cu, Cv = get_interp_arrays_2D(t, x, y, U, V)
+    assert relerr(Cu, oracle.prefilter3(U)) < 1e-13
+    assert relerr(Cv, oracle.prefilter3(V)) < 1e-13
+
+
+def test_spline_velocity_literals(nb, literals):
+    """Tricubic evaluation at 27 in-grid points (tests/test_flows.py:208-258)."""
+    from numbacs_amd.flows import flow_velocity, get_flow_2D, get_interp_arrays_2D
+
+    t = np.array([0.0, 0.1, 0.2])
+    x = np.array([0.0, 0.5, 1.0])
+    T_, X, Y = np.meshgrid(t, x, x, indexing="ij")
+    u = np.sin(X) * np.cos(Y) + np.sin(T_)
+    v = np.cos(X) * np.sin(Y) + np.cos(T_)
+    grid, Cu, Cv = get_interp_arrays_2D(t, x, x, u, v)
+    f = get_flow_2D(grid, Cu, Cv)
+    ti = np.array([0.05, 0.12, 0.18])
+    xi = np.array([0.1, 0.4, 0.7])
+    Ti, Xi, Yi = np.meshgrid(ti, xi, xi, indexing="ij")
+    vel = flow_velocity(f, Ti.ravel(), np.column_stack((Xi.ravel(), Yi.ravel())), [1.0])
+    assert np.allclose(vel.T, literals["spline3_vel"])
+
+
+@pytest.mark.parametrize("extrap", ["constant", "linear"])
+def test_spline_velocity_out_of_grid_vs_oracle(nb, extrap):
+    """Extrapolated values follow the oracle's stated semantics (the
+    reference's out-of-grid behaviour is unpinned)."""
+    from numbacs_amd.flows import flow_velocity, get_flow_2D, get_interp_arrays_2D
+
+    rng = np.random.default_rng(5)
+    t, x, y = np.arange(6.0), np.linspace(0, 1, 9), np.linspace(-1, 1, 7)
+    U = rng.standard_normal((6, 9, 7))
+    V = rng.standard_normal((6, 9, 7))
+    grid, Cu, Cv = get_interp_arrays_2D(t, x, y, U, V)
+    f = get_flow_2D(grid, Cu, Cv, extrap_mode=extrap)
+    n = 500
+    tq = rng.uniform(-1, 6, n)
+    pq = np.column_stack((rng.uniform(-0.3, 1.3, n), rng.uniform(-1.5, 1.5, n)))
+    vel = flow_velocity(f, tq, pq, [1.0])
+    ref = oracle.rhs_eval("spline", np.array([1.0]), tq, pq,
+                          spline=oracle.Spline(grid, Cu, Cv, extrap_mode=extrap))
+    assert relerr(vel, ref) < 1e-12
+
+
+# ------------------------------------------------------------- flow maps --
+
+
+def test_flowmap_grid_2D_fixture(nb, coords_dg, fm_data, mask_dg):
+    """tests/test_integration.py:55-64 of the reference."""
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    f, p, _ = get_predefined_flow("double_gyre")
+    x, y = coords_dg
+    fm = flowmap_grid_2D(f, 0.0, T_FIX, x, y, p)
+    assert np.allclose(fm.astype(np.float32), fm_data)
+    ref, _, _ = oracle.flowmap_grid_2d("double_gyre", p, 0.0, T_FIX, x, y)
+    assert relerr(fm, ref) < 1e-12
+    fmm = flowmap_grid_2D(f, 0.0, T_FIX, x, y, p, mask=mask_dg)
+    assert np.allclose(fmm.astype(np.float32), apply_mask(fm_data, mask_dg))
+    assert (fmm[mask_dg] == 0).all()
+
+
+def test_flowmap_points_fixture(nb, coords_dg, fm_data):
+    """tests/test_integration.py:38-42 of the reference."""
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap
+
+    f, p, _ = get_predefined_flow("double_gyre")
+    x, y = coords_dg
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    pts = np.column_stack((X.ravel(), Y.ravel()))
+    fm = flowmap(f, 0.0, T_FIX, pts, p).reshape(21, 11, 2)
+    assert np.allclose(fm.astype(np.float32), fm_data)
+
+
+@pytest.mark.parametrize("case", ["dg_fwd8", "dg_c1_sub", "dg_t20_sub"])
+def test_flowmap_vs_scipy_dop853(nb, scipy_golden, case):
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    G = scipy_golden
+    f = get_predefined_flow("double_gyre", return_default_params=False, return_domain=False)
+    t0, T = G[case + "_t0T"]
+    fm = flowmap_grid_2D(f, t0, T, G[case + "_x"], G[case + "_y"], G[case + "_params"])
+    assert relerr(fm, G[case + "_fm"]) < 1e-10
+
+
+def test_flowmap_double_gyre_full_vs_oracle(nb):
+    """Double gyre 401x201, t0=0, T=-10 (the CPU-runnable configuration)."""
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    f, p, _ = get_predefined_flow("double_gyre", int_direction=-1.0)
+    x = np.linspace(0, 2, 401)
+    y = np.linspace(0, 1, 201)
+    fm, st = flowmap_grid_2D(f, 0.0, -10.0, x, y, p, return_status=True)
+    ref, st_ref, _ = oracle.flowmap_grid_2d("double_gyre", p, 0.0, -10.0, x, y)
+    assert (st == 1).all() and (st_ref == 1).all()
+    assert relerr(fm, ref) < 1e-7
+    from numbacs_amd.diagnostics import ftle_grid_2D
+
+    ftle = ftle_grid_2D(fm, -10.0, 0.005, 0.005)
+    ftle_ref = oracle.ftle_grid_2d(ref, -10.0, 0.005, 0.005)
+    assert relerr(ftle, ftle_ref) < 1e-6
+
+
+def _conditioned_close(fm, ref, spread, rtol=1e-7, factor=100.0):
+    """|fm - ref| within rtol*scale, or within factor x the oracle's own
+    response to a few-ulp change of the initial condition (chaotic
+    trajectories amplify rounding differences)."""
+    scale = np.abs(ref).max()
+    err = np.abs(fm - ref).max(-1)
+    return np.all(err <= np.maximum(rtol * scale, factor * spread)), err, spread
+
+
+def test_flowmap_bickley_vs_oracle(nb):
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    f, p, _ = get_predefined_flow("bickley_jet")
+    p = p.copy()
+    p[0] = -1.0
+    x = np.linspace(0, math.pi * 6.371, 2001)[::20]
+    y = np.linspace(-3, 3, 801)[::20]
+    fm, st = flowmap_grid_2D(f, 0.0, -40.0, x, y, p, return_status=True)
+    ref, spread = rounding_spread(
+        lambda xx, yy: oracle.flowmap_grid_2d("bickley_jet", p, 0.0, -40.0, xx, yy)[0], x, y)
+    assert (st == 1).all()
+    ok, err, floor = _conditioned_close(fm, ref, spread)
+    assert ok, (err.max(), floor.max())
+    # most trajectories agree to the strict bar
+    assert np.mean(err <= 1e-7 * np.abs(ref).max()) > 0.9
+
+
+def test_flowmap_bickley_vs_scipy(nb, scipy_golden):
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    G = scipy_golden
+    case = "bickley_t40_sub"
+    f = get_predefined_flow("bickley_jet", return_default_params=False, return_domain=False)
+    fm = flowmap_grid_2D(f, 0.0, -40.0, G[case + "_x"], G[case + "_y"], G[case + "_params"])
+    ref = G[case + "_fm"]
+    err = np.abs(fm - ref).max(-1) / np.abs(ref).max()
+    assert np.median(err) < 1e-9
+    assert err.max() < 1e-2
+
+
+@pytest.fixture(scope="module")
+def c2_small():
+    """The gridded-velocity configuration's field with every 4th particle."""
+    from numbacs_amd.workloads import particle_axes, synthetic_velocity
+
+    t, lon, lat, U, V = synthetic_velocity()
+    xp, yp = particle_axes()
+    return t, lon, lat, U, V, xp[::4], yp[::4]
+
+
+def test_flowmap_spline_spherical_vs_oracle(nb, c2_small):
+    from numbacs_amd.flows import get_flow_2D, get_interp_arrays_2D
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    t, lon, lat, U, V, xp, yp = c2_small
+    grid, Cu, Cv = get_interp_arrays_2D(t, lon, lat, U, V)
+    assert relerr(Cu, oracle.prefilter3(U)) < 1e-13
+    f = get_flow_2D(grid, Cu, Cv, spherical=1)
+    fm, st = flowmap_grid_2D(f, 72.0, -72.0, xp, yp, np.array([-1.0]), return_status=True)
+    S = oracle.Spline(grid, Cu, Cv, spherical=1)
+    ref, spread = rounding_spread(
+        lambda xx, yy: oracle.flowmap_grid_2d("spline", np.array([-1.0]), 72.0, -72.0, xx, yy,
+                                              spline=S)[0], xp, yp)
+    assert (st == 1).all()
+    ok, err, floor = _conditioned_close(fm, ref, spread)
+    assert ok, (err.max(), floor.max())
+    assert np.mean(err <= 1e-7 * np.abs(ref).max()) > 0.99
+
+
+def test_flowmap_status_budget(nb):
+    """A step budget of 1 stops every solve with status -2 (Hairer's NMAX)."""
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    f, p, _ = get_predefined_flow("double_gyre")
+    x, y = np.linspace(0, 2, 9), np.linspace(0, 1, 5)
+    mask = np.zeros((9, 5), bool)
+    mask[0, 0] = True
+    fm, st = flowmap_grid_2D(f, 0.0, 8.0, x, y, p, mask=mask, max_steps=1, return_status=True)
+    ref, st_ref, _ = oracle.flowmap_grid_2d("double_gyre", p, 0.0, 8.0, x, y, mask=mask, nmax=1)
+    assert st[0, 0] == 0 and (st.ravel()[1:] == -2).all()
+    assert (st == st_ref).all()
+    assert relerr(fm, ref) < 1e-12
+
+
+def test_flowmap_empty(nb):
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap, flowmap_grid_2D
+
+    f, p, _ = get_predefined_flow("double_gyre")
+    assert flowmap_grid_2D(f, 0.0, 1.0, np.zeros(0), np.linspace(0, 1, 4), p).shape == (0, 4, 2)
+    assert flowmap(f, 0.0, 1.0, np.zeros((0, 2)), p).shape == (0, 2)
+
+
+# ------------------------------------------------------------ FTLE / eig --
+
+
+def test_ftle_fixture(nb, coords_dg, fm_data, ftle_data, mask_dg):
+    """tests/test_diagnostics.py:82-96 of the reference."""
+    from numbacs_amd.diagnostics import ftle_grid_2D
+
+    x, y = coords_dg
+    ftle = ftle_grid_2D(fm_data, T_FIX, x[1], y[1])
+    assert np.allclose(ftle.astype(np.float32), ftle_data)
+    assert relerr(ftle, oracle.ftle_grid_2d(fm_data, T_FIX, x[1], y[1])) < 1e-13
+    ftle_m = ftle_grid_2D(fm_data, T_FIX, x[1], y[1], mask=mask_dg)
+    assert np.allclose(ftle_m.astype(np.float32), apply_mask(ftle_data, mask_dg))
+
+
+def test_C_eig_fixture(nb, coords_dg, fm_data, C_eig_data):
+    """tests/test_diagnostics.py:133-150 of the reference (reconstruction)."""
+    from numbacs_amd.diagnostics import C_eig_2D
+
+    x, y = coords_dg
+    ev, V = C_eig_2D(fm_data, x[1], y[1])
+    ev_ref, V_ref = C_eig_data
+    assert np.allclose(ev.astype(np.float32), ev_ref)
+    assert np.allclose(reconstruct(ev, V).astype(np.float32),
+                       reconstruct(ev_ref.astype(np.float64), V_ref.astype(np.float64)).astype(
+                           np.float32))
+    ev_o, V_o = oracle.C_eig_2d(fm_data, x[1], y[1])
+    assert relerr(ev, ev_o) < 1e-13
+    assert relerr(V, V_o) < 1e-12
+
+
+def test_ftle_eig_fused_matches_separate(nb):
+    from numbacs_amd import _lib
+    from numbacs_amd.diagnostics import C_eig_2D, ftle_eig_2d_device, ftle_from_eig
+
+    rng = np.random.default_rng(11)
+    fm = np.cumsum(rng.standard_normal((67, 45, 2)), axis=0)
+    ev, V = C_eig_2D(fm, 0.3, 0.2)
+    fd, ed = ftle_eig_2d_device(_lib.to_dev(fm), -5.0, 0.3, 0.2)
+    assert relerr(_lib.to_host(fd), ftle_from_eig(ev[:, :, 1], -5.0)) < 1e-14
+    assert np.array_equal(_lib.to_host(ed), V[:, :, :, 1])
+
+
+# ---------------------------------------------------------------- ridges --
+
+
+def test_ridge_pts_fixture(nb, coords_dg, ftle_data, C_eig_data, ridge_pts_data):
+    """tests/test_extraction.py:6-12 of the reference."""
+    from numbacs_amd.extraction import ftle_ridge_pts
+
+    x, y = coords_dg
+    _, evecs = C_eig_data
+    pts = ftle_ridge_pts(ftle_data, evecs[:, :, :, 1], x, y)
+    assert pts.shape == ridge_pts_data.shape
+    assert np.allclose(pts, ridge_pts_data)
+    ref = oracle.ridge_pts(ftle_data, evecs[:, :, :, 1], x, y)
+    assert relerr(pts, ref) < 1e-14
+
+
+def test_ridges_fixture_grouping(nb, coords_dg, ftle_data, C_eig_data):
+    """ftle_ridges on the fixture world: grouping equals scipy.ndimage.label's
+    (the reference's dependency) applied to the oracle's ridge flags."""
+    from numbacs_amd.extraction import ftle_ridges
+
+    x, y = coords_dg
+    _, evecs = C_eig_data
+    ridges = ftle_ridges(ftle_data, evecs[:, :, :, 1], x, y)
+    pts, rb = oracle.ridge_predicate(ftle_data, evecs[:, :, :, 1], x, y)
+    lab, n = ndimage.label(rb, structure=ndimage.generate_binary_structure(2, 2))
+    expect = []
+    for i in range(1, n + 1):
+        inds = np.nonzero(lab.ravel() == i)[0]
+        if len(inds) >= 3:
+            expect.append(pts[inds])
+    assert [len(r) for r in ridges] == [len(r) for r in expect] == [6, 9, 3]
+    assert all(np.array_equal(a, b) for a, b in zip(ridges, expect))
+
+
+@pytest.mark.parametrize("q", [0, 50, 90, 99.5, 100])
+def test_percentile_bit_exact(nb, q):
+    from numbacs_amd import _lib
+
+    rng = np.random.default_rng(int(q * 10))
+    f = rng.standard_normal(100003) * 3
+    f[::7] = 0.0  # ties, like the zero border of an FTLE field
+    f[5] = -0.0
+    fd = _lib.to_dev(f)
+    out = _lib.empty((1,))
+    _lib.check(_lib.load().nbx_percentile(_lib.ptr(fd), f.size, float(q), _lib.ptr(out),
+                                          _lib.stream_ptr()))
+    got = _lib.to_host(out)[0]
+    assert got == np.percentile(f, q)
+    assert got == oracle.percentile(f, q)
+
+
+def test_percentile_nan(nb):
+    from numbacs_amd import _lib
+
+    f = np.arange(1000, dtype=np.float64)
+    f[17] = np.nan
+    fd = _lib.to_dev(f)
+    out = _lib.empty((1,))
+    _lib.check(_lib.load().nbx_percentile(_lib.ptr(fd), f.size, 90.0, _lib.ptr(out),
+                                          _lib.stream_ptr()))
+    assert np.isnan(_lib.to_host(out)[0]) and np.isnan(np.percentile(f, 90))
+
+
+@pytest.mark.parametrize("shape,density", [((1, 1), 1.0), ((3, 200), 0.5), ((257, 131), 0.35),
+                                           ((1000, 37), 0.6)])
+def test_label8_vs_scipy(nb, shape, density):
+    from numbacs_amd.extraction import label8
+
+    rng = np.random.default_rng(shape[0])
+    b = rng.random(shape) < density
+    lab, n = label8(b)
+    lab_ref, n_ref = ndimage.label(b, structure=ndimage.generate_binary_structure(2, 2))
+    assert n == n_ref
+    assert np.array_equal(lab, lab_ref)
+
+
+def _field_and_evec(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0, 3, nx)
+    y = np.linspace(-1, 1, ny)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    f = np.zeros((nx, ny))
+    for _ in range(12):
+        a, b, c = rng.uniform(0.5, 6, 3)
+        f += np.exp(-((np.sin(a * X + c) - np.cos(b * Y)) ** 2) * 8)
+    th = rng.uniform(0, np.pi, (nx, ny))
+    ev = np.stack((np.cos(th), np.sin(th)), -1)
+    f[0, :] = f[-1, :] = f[:, 0] = f[:, -1] = 0
+    return f, ev, x, y
+
+
+@pytest.mark.parametrize("percentile", [0, 90])
+def test_ridges_vs_oracle_random(nb, percentile):
+    from numbacs_amd.extraction import ftle_ridge_pts, ftle_ridges
+
+    f, ev, x, y = _field_and_evec(311, 173, 7)
+    pts = ftle_ridge_pts(f, ev, x, y, percentile=percentile)
+    ref = oracle.ridge_pts(f, ev, x, y, percentile_=percentile)
+    assert pts.shape == ref.shape and len(ref) > 100
+    assert relerr(pts, ref) < 1e-14
+    ridges = ftle_ridges(f, ev, x, y, percentile=percentile, min_ridge_pts=3)
+    ridges_ref = oracle.ridges(f, ev, x, y, percentile_=percentile, min_ridge_pts=3)
+    assert len(ridges) == len(ridges_ref) > 3
+    for a, b in zip(ridges, ridges_ref):
+        assert a.shape == b.shape and np.allclose(a, b, rtol=0, atol=1e-14)
+
+
+def test_ridges_empty_and_degenerate(nb):
+    from numbacs_amd.extraction import ftle_ridge_pts, ftle_ridges
+
+    f = np.zeros((6, 5))
+    ev = np.zeros((6, 5, 2))
+    x, y = np.arange(6.0), np.arange(5.0)
+    assert ftle_ridge_pts(f, ev, x, y).shape == (0, 2)
+    assert ftle_ridges(f, ev, x, y) == []
+    assert ftle_ridges(f, ev, x, y, percentile=90) == []

@@ -1,0 +1,243 @@
+"""The CPU oracle against the reference's fixtures and literals, scipy's
+DOP853 (Hairer's Fortran code), numpy.percentile and scipy.ndimage.label --
+the checks that pin it before it is trusted as the checker of the HIP path.
+Planted mistakes (in the oracle's inputs, on the CPU) show the checks bite.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import oracle
+from conftest import apply_mask, reconstruct, rounding_spread
+
+DG = "double_gyre"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    oracle.lib()
+
+
+def dg_params(direction=1.0):
+    return np.array([direction, 0.1, 0.25, 0.0, 0.2 * math.pi, 0.0])
+
+
+def test_flowmap_fixture_bit_exact_f32(coords_dg, fm_data, mask_dg):
+    """fm.npy (tests/test_integration.py:55-64): all 462 values equal after
+    the float32 cast."""
+    x, y = coords_dg
+    fm, st, ns = oracle.flowmap_grid_2d(DG, dg_params(), 0.0, 8.0, x, y)
+    assert (st == 1).all()
+    assert (fm.astype(np.float32) == fm_data).all()
+    fmm, stm, _ = oracle.flowmap_grid_2d(DG, dg_params(), 0.0, 8.0, x, y, mask=mask_dg)
+    assert np.allclose(fmm.astype(np.float32), apply_mask(fm_data, mask_dg))
+    assert (stm[mask_dg] == 0).all()
+
+
+@pytest.mark.parametrize("case", ["dg_fwd8", "dg_c1_sub", "dg_t20_sub"])
+def test_flowmap_vs_scipy_dop853(scipy_golden, case):
+    G = scipy_golden
+    t0, T = G[case + "_t0T"]
+    fm, _, _ = oracle.flowmap_grid_2d(DG, G[case + "_params"], t0, T, G[case + "_x"],
+                                      G[case + "_y"])
+    ref = G[case + "_fm"]
+    assert np.abs(fm - ref).max() / np.abs(ref).max() < 1e-11
+
+
+def test_flowmap_bickley_vs_scipy_dop853(scipy_golden):
+    """Bickley jet over 40 days is chaotic: most trajectories agree to
+    rounding, the rest within the spread a 1-ulp change of the initial
+    condition produces."""
+    G = scipy_golden
+    c = "bickley_t40_sub"
+    x, y, p = G[c + "_x"], G[c + "_y"], G[c + "_params"]
+    ref = G[c + "_fm"]
+
+    def run(xx, yy):
+        return oracle.flowmap_grid_2d("bickley_jet", p, 0.0, -40.0, xx, yy)[0]
+
+    fm, spread = rounding_spread(run, x, y)
+    scale = np.abs(ref).max()
+    err = np.abs(fm - ref).max(-1)
+    assert np.median(err) < 1e-10 * scale
+    assert np.all(err <= np.maximum(1e-9 * scale, 100 * spread))
+
+
+def test_planted_tableau_error_is_caught(scipy_golden):
+    """A wrong direction sign (planted in the oracle's input) breaks the
+    agreement with scipy by orders of magnitude."""
+    G = scipy_golden
+    c = "dg_c1_sub"
+    p = G[c + "_params"].copy()
+    p[4] *= 1.0 + 1e-9   # perturb omega at the 1e-9 level
+    fm, _, _ = oracle.flowmap_grid_2d(DG, p, 0.0, -10.0, G[c + "_x"], G[c + "_y"])
+    assert np.abs(fm - G[c + "_fm"]).max() > 1e-9
+
+
+def test_velocity_literals(literals):
+    ti = np.array([0.05, 0.12, 0.18])
+    xi = np.array([0.1, 0.4, 0.7])
+    Ti, Xi, Yi = np.meshgrid(ti, xi, xi, indexing="ij")
+    pts = np.column_stack((Xi.ravel(), Yi.ravel()))
+    v = oracle.rhs_eval(DG, dg_params(), Ti.ravel(), pts)
+    assert np.allclose(v.T, literals["dg_vel"])
+    r_e = 6371.0e-3
+    U0 = 86400 * 62.66e-6
+    c2, c3 = 0.205 * U0, 0.461 * U0
+    pb = np.array([1.0, U0, 1.77, 0.0075, 0.15, 0.3, 2 / r_e, 4 / r_e, 6 / r_e,
+                   c3 + (math.sqrt(5) - 1) * (c2 - c3), c2, c3])
+    v = oracle.rhs_eval("bickley_jet", pb, Ti.ravel(), pts)
+    assert np.allclose(v.T, literals["bickley_vel"])
+
+
+def _literal_field():
+    t = np.array([0.0, 0.1, 0.2])
+    x = np.array([0.0, 0.5, 1.0])
+    T_, X, Y = np.meshgrid(t, x, x, indexing="ij")
+    return t, x, np.sin(X) * np.cos(Y) + np.sin(T_), np.cos(X) * np.sin(Y) + np.cos(T_)
+
+
+def test_prefilter_literals(literals):
+    t, x, u, v = _literal_field()
+    assert np.allclose(oracle.prefilter3(u), literals["prefilter3_Cu"])
+    assert np.allclose(oracle.prefilter3(v), literals["prefilter3_Cv"])
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    assert np.allclose(oracle.prefilter2(np.sin(X) * np.cos(Y)), literals["prefilter2_Cu"])
+    assert np.allclose(oracle.prefilter2(np.cos(X) * np.sin(Y)), literals["prefilter2_Cv"])
+
+
+def test_prefilter_interpolates_and_is_natural():
+    """The spline reproduces the samples at the nodes and has zero second
+    derivative at the ends of each axis."""
+    rng = np.random.default_rng(1)
+    U = rng.standard_normal((9, 13, 11))
+    C = oracle.prefilter3(U)
+    grid = ((0.0, 8.0, 9), (0.0, 1.2, 13), (-1.0, 1.0, 11))
+    T_, X, Y = np.meshgrid(np.linspace(0, 8, 9), np.linspace(0, 1.2, 13), np.linspace(-1, 1, 11),
+                           indexing="ij")
+    vals = oracle.spline3_eval(grid, C, np.column_stack((T_.ravel(), X.ravel(), Y.ravel())))
+    assert np.abs(vals - U.ravel()).max() < 1e-12
+    d2 = C[:, :, 0] - 2 * C[:, :, 1] + C[:, :, 2]
+    assert np.abs(d2).max() < 1e-12
+
+
+def test_spline_eval_literals(literals):
+    t, x, u, v = _literal_field()
+    grid = ((0.0, 0.2, 3), (0.0, 1.0, 3), (0.0, 1.0, 3))
+    ti = np.array([0.05, 0.12, 0.18])
+    xi = np.array([0.1, 0.4, 0.7])
+    Ti, Xi, Yi = np.meshgrid(ti, xi, xi, indexing="ij")
+    pts = np.column_stack((Ti.ravel(), Xi.ravel(), Yi.ravel()))
+    assert np.allclose(oracle.spline3_eval(grid, oracle.prefilter3(u), pts),
+                       literals["spline3_vel"][0])
+    assert np.allclose(oracle.spline3_eval(grid, oracle.prefilter3(v), pts),
+                       literals["spline3_vel"][1])
+
+
+def test_spline_reproduces_cubics():
+    """A cubic polynomial in each variable is reproduced exactly in-grid
+    only if it has zero second derivative at the ends -- a linear function
+    is, to rounding, everywhere in the box."""
+    t, x, y = np.linspace(0, 3, 7), np.linspace(-1, 1, 9), np.linspace(0, 2, 8)
+    T_, X, Y = np.meshgrid(t, x, y, indexing="ij")
+    F = 0.3 * T_ - 1.7 * X + 2.5 * Y + 0.25
+    C = oracle.prefilter3(F)
+    rng = np.random.default_rng(2)
+    q = np.column_stack((rng.uniform(0, 3, 300), rng.uniform(-1, 1, 300), rng.uniform(0, 2, 300)))
+    grid = ((0.0, 3.0, 7), (-1.0, 1.0, 9), (0.0, 2.0, 8))
+    exact = 0.3 * q[:, 0] - 1.7 * q[:, 1] + 2.5 * q[:, 2] + 0.25
+    assert np.abs(oracle.spline3_eval(grid, C, q) - exact).max() < 1e-12
+    # constant extrapolation clamps; linear extrapolation continues the plane
+    qo = q + np.array([4.0, 0.0, 0.0])
+    clamp = 0.3 * 3.0 - 1.7 * q[:, 1] + 2.5 * q[:, 2] + 0.25
+    assert np.abs(oracle.spline3_eval(grid, C, qo, "constant") - clamp).max() < 1e-12
+    lin = 0.3 * qo[:, 0] - 1.7 * q[:, 1] + 2.5 * q[:, 2] + 0.25
+    assert np.abs(oracle.spline3_eval(grid, C, qo, "linear") - lin).max() < 1e-11
+
+
+def test_ftle_fixture(coords_dg, fm_data, ftle_data, mask_dg):
+    x, y = coords_dg
+    f = oracle.ftle_grid_2d(fm_data, 8.0, x[1], y[1])
+    assert np.allclose(f.astype(np.float32), ftle_data)
+    fmk = oracle.ftle_grid_2d(fm_data, 8.0, x[1], y[1], mask=mask_dg)
+    assert np.allclose(fmk.astype(np.float32), apply_mask(ftle_data, mask_dg))
+
+
+def test_gradient_and_eig_literals(coords_dg, fm_data, literals):
+    """gradF_stencil_2D at (5, 5) and eigvalsh_max_2D literals
+    (tests/test_utils.py:74-88,133-141): an identity flow map shifted by a
+    linear map A has FTLE log(max eig(A^T A))/(2|T|)."""
+    B = np.array([[1.3, -2.4], [0.7, 5.2]])
+    x = np.linspace(0, 1, 5)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    fm = np.stack((B[0, 0] * X + B[0, 1] * Y, B[1, 0] * X + B[1, 1] * Y), -1)
+    ev, V = oracle.C_eig_2d(fm, x[1], x[1])
+    assert np.isclose(ev[2, 2, 1], literals["eigvalsh_max"])
+    ftle = oracle.ftle_grid_2d(fm, 2.0, x[1], x[1])
+    assert np.isclose(ftle[2, 2], math.log(float(literals["eigvalsh_max"])) / 4)
+    xs, ys = coords_dg
+    g = literals["gradF_5_5"]
+    dxdx = (fm_data[6, 5, 0] - fm_data[4, 5, 0]) / (2 * xs[1])
+    assert np.isclose(dxdx, g[0])
+    C = np.array([[g[0] ** 2 + g[2] ** 2, g[0] * g[1] + g[2] * g[3]],
+                  [g[0] * g[1] + g[2] * g[3], g[1] ** 2 + g[3] ** 2]])
+    f = oracle.ftle_grid_2d(fm_data, 8.0, xs[1], ys[1])
+    assert np.isclose(f[5, 5], math.log(np.linalg.eigvalsh(C)[1]) / 16, rtol=1e-6)
+
+
+def test_C_eig_fixture(coords_dg, fm_data, C_eig_data):
+    x, y = coords_dg
+    ev, V = oracle.C_eig_2d(fm_data, x[1], y[1])
+    ev_ref, V_ref = C_eig_data
+    assert np.allclose(ev.astype(np.float32), ev_ref)
+    R = reconstruct(ev, V)
+    assert np.allclose(R.astype(np.float32), reconstruct(ev_ref.astype(np.float64),
+                                                         V_ref.astype(np.float64)).astype(
+                                                             np.float32))
+
+
+def test_eig_vs_numpy_eigh():
+    rng = np.random.default_rng(4)
+    fm = rng.standard_normal((40, 30, 2)).cumsum(0).cumsum(1)
+    ev, V = oracle.C_eig_2d(fm, 0.1, 0.2)
+    F = fm
+    gx = (F[2:, 1:-1] - F[:-2, 1:-1]) / 0.2
+    gy = (F[1:-1, 2:] - F[1:-1, :-2]) / 0.4
+    J = np.stack((np.stack((gx[..., 0], gy[..., 0]), -1), np.stack((gx[..., 1], gy[..., 1]), -1)),
+                 -2)
+    C = np.swapaxes(J, -1, -2) @ J
+    w, _ = np.linalg.eigh(C)
+    assert np.allclose(ev[1:-1, 1:-1], w, rtol=1e-10, atol=1e-12)
+    Cr = reconstruct(ev[1:-1, 1:-1], V[1:-1, 1:-1])
+    assert np.allclose(Cr, C, rtol=1e-10, atol=1e-10)
+
+
+def test_ridge_pts_fixture(coords_dg, ftle_data, C_eig_data, ridge_pts_data):
+    x, y = coords_dg
+    pts = oracle.ridge_pts(ftle_data, C_eig_data[1][:, :, :, 1], x, y)
+    assert pts.shape == ridge_pts_data.shape and np.allclose(pts, ridge_pts_data)
+
+
+def test_ridges_fixture_counts(coords_dg, ftle_data, C_eig_data):
+    x, y = coords_dg
+    r = oracle.ridges(ftle_data, C_eig_data[1][:, :, :, 1], x, y)
+    assert [len(a) for a in r] == [6, 9, 3]
+
+
+@pytest.mark.parametrize("q", [0, 10, 50, 90, 99.9, 100])
+def test_percentile_vs_numpy(q):
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(10007)
+    f[::5] = 0.0
+    assert oracle.percentile(f, q) == np.percentile(f, q)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_label8_vs_scipy(seed):
+    rng = np.random.default_rng(seed)
+    b = rng.random((120, 77)) < 0.45
+    lab, n = oracle.label8(b)
+    ref, nref = ndimage.label(b, structure=ndimage.generate_binary_structure(2, 2))
+    assert n == nref and np.array_equal(lab, ref)
