@@ -635,31 +635,48 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
     int64_t N = kSortChunk;
     while (N < n) N <<= 1;
-    const size_t bytes = align256(sizeof(SelState)) + align256(4 * n) * 9 + align256(16 * n) +
-                         align256(4 * (tiles + 1)) + align256(4) * 6 + align256(8 * N) +
-                         align256(4 * (n + 1));
+    struct Layout {
+        SelState* st;
+        int32_t *flag, *pos, *tmp, *count, *par, *idx, *seg_start, *seg_ex, *seg_first, *kept_len,
+            *kept_flag, *out_off, *ridge_idx, *nseg, *total, *nridges;
+        double2* dense;
+        unsigned long long* keys;
+        void carve(Arena& A, int64_t n, int64_t tiles, int64_t N) {
+            st = A.take<SelState>(1);
+            flag = A.take<int32_t>(n);
+            pos = A.take<int32_t>(n);
+            dense = A.take<double2>(n);
+            tmp = A.take<int32_t>(tiles + 1);
+            count = A.take<int32_t>(1);
+            par = A.take<int32_t>(n);
+            idx = A.take<int32_t>(n);
+            keys = A.take<unsigned long long>(N);
+            seg_start = A.take<int32_t>(n);
+            seg_ex = A.take<int32_t>(n);
+            seg_first = A.take<int32_t>(n + 1);
+            kept_len = A.take<int32_t>(n);
+            kept_flag = A.take<int32_t>(n);
+            out_off = A.take<int32_t>(n);
+            ridge_idx = A.take<int32_t>(n);
+            nseg = A.take<int32_t>(1);
+            total = A.take<int32_t>(1);
+            nridges = A.take<int32_t>(1);
+        }
+    } L;
+    Arena measure{nullptr, 0};
+    L.carve(measure, n, tiles, N);
     int err = 0;
-    Arena A{static_cast<char*>(workspace(1, bytes, &err)), 0};
+    Arena A{static_cast<char*>(workspace(1, measure.used, &err)), 0};
     if (err) return err;
-    auto* st = A.take<SelState>(1);
-    auto* flag = A.take<int32_t>(n);
-    auto* pos = A.take<int32_t>(n);
-    auto* dense = A.take<double2>(n);
-    auto* tmp = A.take<int32_t>(tiles + 1);
-    auto* count = A.take<int32_t>(1);
-    auto* par = A.take<int32_t>(n);
-    auto* idx = A.take<int32_t>(n);
-    auto* keys = A.take<unsigned long long>(N);
-    auto* seg_start = A.take<int32_t>(n);
-    auto* seg_ex = A.take<int32_t>(n);
-    auto* seg_first = A.take<int32_t>(n + 1);
-    auto* kept_len = A.take<int32_t>(n);
-    auto* kept_flag = A.take<int32_t>(n);
-    auto* out_off = A.take<int32_t>(n);
-    auto* ridge_idx = A.take<int32_t>(n);
-    auto* nseg = A.take<int32_t>(1);
-    auto* total = A.take<int32_t>(1);
-    auto* nridges = A.take<int32_t>(1);
+    L.carve(A, n, tiles, N);
+    SelState* st = L.st;
+    int32_t *flag = L.flag, *pos = L.pos, *tmp = L.tmp, *count = L.count, *par = L.par,
+            *idx = L.idx, *seg_start = L.seg_start, *seg_ex = L.seg_ex,
+            *seg_first = L.seg_first, *kept_len = L.kept_len, *kept_flag = L.kept_flag,
+            *out_off = L.out_off, *ridge_idx = L.ridge_idx, *nseg = L.nseg, *total = L.total,
+            *nridges = L.nridges;
+    double2* dense = L.dense;
+    unsigned long long* keys = L.keys;
 
     if (int rc = run_predicate(f, evec, x, y, nx, ny, dx, dy, sdd_thresh, percentile, st, flag,
                                dense, s))
