@@ -10,7 +10,7 @@ import pytest
 from scipy import ndimage
 
 import oracle
-from conftest import apply_mask, reconstruct, rounding_spread
+from conftest import apply_mask, flowmap_agreement, ic_sensitivity, reconstruct
 
 DG = "double_gyre"
 
@@ -58,11 +58,11 @@ def test_flowmap_bickley_vs_scipy_dop853(scipy_golden):
     def run(xx, yy):
         return oracle.flowmap_grid_2d("bickley_jet", p, 0.0, -40.0, xx, yy)[0]
 
-    fm, spread = rounding_spread(run, x, y)
-    scale = np.abs(ref).max()
-    err = np.abs(fm - ref).max(-1)
-    assert np.median(err) < 1e-10 * scale
-    assert np.all(err <= np.maximum(1e-9 * scale, 100 * spread))
+    fm, sens = ic_sensitivity(run, x, y)
+    ok, frac, err = flowmap_agreement(fm, ref, sens)
+    assert ok
+    assert frac > 0.8
+    assert np.median(err) < 1e-10 * np.abs(ref).max()
 
 
 def test_planted_tableau_error_is_caught(scipy_golden):
