@@ -95,6 +95,15 @@ int nbx_flowmap_pts(const nbx_flow* flow, double t0, double T, const double* pts
                     const uint8_t* mask, int64_t max_steps, int precision, double* out,
                     int32_t* status, int32_t* nsteps, nbx_stream stream);
 
+/* Launch shape of the flow-map kernels for one (flow kind, precision):
+ * lanes_per_particle (1, 2 or 4; spline flows only: the lanes of a group
+ * split each tricubic gather) and threads per block (64, 128 or 256).
+ * Process-wide; set by the autotuner (numbacs_amd.autotune).  No reference
+ * counterpart (the reference's parallel loop has no tunables,
+ * integration.py:166). */
+int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int block_threads);
+int nbx_get_launch_config(int kind, int precision, int* lanes_per_particle, int* block_threads);
+
 /* --- FTLE and Cauchy-Green eigen --------------------------------------- */
 
 /* ftle_grid_2D (diagnostics.py:21-65): interior central differences,

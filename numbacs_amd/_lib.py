@@ -40,6 +40,8 @@ SIGNATURES = {
                                    _c_d, _c_p, _c_i64, _c_i, _c_p, _c_p, _c_p, _c_p]),
     "nbx_flowmap_pts": (_c_i, [_c_p, _c_d, _c_d, _c_p, _c_i64, _c_p, _c_i, _c_d, _c_d, _c_p,
                                _c_i64, _c_i, _c_p, _c_p, _c_p, _c_p]),
+    "nbx_set_launch_config": (_c_i, [_c_i, _c_i, _c_i, _c_i]),
+    "nbx_get_launch_config": (_c_i, [_c_i, _c_i, _c_p, _c_p]),
     "nbx_ftle_grid_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_d, _c_p, _c_p, _c_p]),
     "nbx_C_eig_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_p, _c_p, _c_p, _c_p]),
     "nbx_ftle_eig_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_d, _c_p, _c_p, _c_p, _c_p]),

@@ -1,4 +1,6 @@
-// DOP853 for a two-component state, one solve per GPU thread.
+// DOP853 for a two-component state, one solve per GPU thread (or per group
+// of G cooperating lanes, see flows.h), written as an init + step state
+// machine so that a persistent kernel can refill finished lanes.
 //
 // Method: Dormand-Prince 8(5,3) as published by Hairer, Norsett & Wanner
 // ("Solving Ordinary Differential Equations I", 2nd ed., sec. II.10), with
@@ -8,6 +10,17 @@
 //   hmax = |xend - x|, last step clipped to xend, blended err5/err3 norm.
 // Stiffness detection (every 1000 accepted steps) and dense output are not
 // part of the flow-map path and are omitted.
+//
+// Precision forms (R = stage precision):
+//   R = double: the state is the absolute position (anchor 0), every
+//               operation in the order of Hairer's code;
+//   R = float : the state is a float displacement from a double anchor (the
+//               initial position) and the stored stages and the velocity
+//               evaluation (spline gathers and sums) are float, while time,
+//               step size, the stage sums, the error norm and the step-size
+//               control stay double.  Storing displacement keeps the float
+//               rounding relative to the distance travelled, not to
+//               |position|.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -109,142 +122,192 @@ constexpr double expo1 = 1.0 / 8.0;
 
 }  // namespace dop
 
-struct SolveResult {
-    int idid;     // 1 ok, -2 step budget exhausted, -3 step size underflow
-    int nstep;    // attempted steps
+// Per-solve state.  Position = (ax, ay) + (dx, dy).
+template <class R>
+struct Dop853State {
+    double x, xend, h, hmax, posneg, facold;
+    double ax, ay;
+    R dx, dy;
+    R k1x, k1y;
+    int nstep;
+    bool last, reject;
+    int idid;  // 0 while running; 1 ok, -2 step budget exhausted, -3 step underflow
+
+    __device__ __forceinline__ double px() const { return ax + static_cast<double>(dx); }
+    __device__ __forceinline__ double py() const { return ay + static_cast<double>(dy); }
 };
 
-// Initial step size (Hairer's HINIT, order 8).
-template <class Flow>
-__device__ __forceinline__ double dop853_hinit(const Flow& f, double x, double y0, double y1,
-                                               double posneg, double f00, double f01,
-                                               double hmax, double rtol, double atol) {
-    double sk0 = atol + rtol * fabs(y0);
-    double sk1 = atol + rtol * fabs(y1);
+// Start a solve at (x0, y0) from time x to xend: first derivative and the
+// initial step (Hairer's HINIT, order 8).
+template <class R, class Flow>
+__device__ __forceinline__ void dop853_init(const Flow& f, Dop853State<R>& s, double x,
+                                            double xend, double p0x, double p0y, double rtol,
+                                            double atol) {
+    s.x = x;
+    s.xend = xend;
+    s.posneg = (xend - x) >= 0 ? 1.0 : -1.0;
+    s.hmax = fabs(xend - x);
+    s.facold = 1.0e-4;
+    s.last = false;
+    s.reject = false;
+    s.nstep = 0;
+    s.idid = 0;
+    if constexpr (sizeof(R) == sizeof(double)) {
+        s.ax = 0.0;
+        s.ay = 0.0;
+        s.dx = p0x;
+        s.dy = p0y;
+    } else {
+        s.ax = p0x;
+        s.ay = p0y;
+        s.dx = R(0);
+        s.dy = R(0);
+    }
+    const double y0 = s.px(), y1 = s.py();
+    f(x, y0, y1, s.k1x, s.k1y);
+    const double f00 = static_cast<double>(s.k1x), f01 = static_cast<double>(s.k1y);
+    const double sk0 = atol + rtol * fabs(y0);
+    const double sk1 = atol + rtol * fabs(y1);
     const double dnf = (f00 / sk0) * (f00 / sk0) + (f01 / sk1) * (f01 / sk1);
     const double dny = (y0 / sk0) * (y0 / sk0) + (y1 / sk1) * (y1 / sk1);
     double h = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
-    h = fmin(h, hmax);
-    h = posneg >= 0 ? fabs(h) : -fabs(h);
-    double f10, f11;
+    h = fmin(h, s.hmax);
+    h = s.posneg >= 0 ? fabs(h) : -fabs(h);
+    R f10, f11;
     f(x + h, y0 + h * f00, y1 + h * f01, f10, f11);
-    const double d0 = (f10 - f00) / sk0;
-    const double d1 = (f11 - f01) / sk1;
-    double der2 = sqrt(d0 * d0 + d1 * d1) / h;
+    const double d0 = (static_cast<double>(f10) - f00) / sk0;
+    const double d1 = (static_cast<double>(f11) - f01) / sk1;
+    const double der2 = sqrt(d0 * d0 + d1 * d1) / h;
     const double der12 = fmax(fabs(der2), sqrt(dnf));
-    const double h1 = der12 <= 1.0e-15 ? fmax(1.0e-6, fabs(h) * 1.0e-3) : pow(0.01 / der12, 0.125);
-    h = fmin(fmin(100 * fabs(h), h1), hmax);
-    return posneg >= 0 ? h : -h;
+    const double h1 =
+        der12 <= 1.0e-15 ? fmax(1.0e-6, fabs(h) * 1.0e-3) : pow(0.01 / der12, 0.125);
+    h = fmin(fmin(100 * fabs(h), h1), s.hmax);
+    s.h = s.posneg >= 0 ? h : -h;
 }
 
-// Integrate y' = f(x, y) from x to xend; (y0, y1) in/out.
-template <class Flow>
-__device__ SolveResult dop853_solve(const Flow& f, double x, const double xend, double& Y0,
-                                    double& Y1, const double rtol, const double atol,
-                                    const long long nmax) {
+// One attempted step (accepted or rejected).  Sets s.idid != 0 when the
+// solve has ended; returns true while it is still running.
+template <class R, class Flow>
+__device__ __forceinline__ bool dop853_step(const Flow& f, Dop853State<R>& s, double rtol,
+                                            double atol, long long nmax) {
     using namespace dop;
-    const double posneg = (xend - x) >= 0 ? 1.0 : -1.0;
-    const double hmax = fabs(xend - x);
-    double facold = 1.0e-4;
-    bool last = false, reject = false;
-    long long nstep = 0;
-    int idid = 1;
-    double y0 = Y0, y1 = Y1;
-    double k1x, k1y;
-    f(x, y0, y1, k1x, k1y);
-    double h = dop853_hinit(f, x, y0, y1, posneg, k1x, k1y, hmax, rtol, atol);
-
-    for (;;) {
-        if (nstep > nmax) { idid = -2; break; }
-        if (0.1 * fabs(h) <= fabs(x) * uround) { idid = -3; break; }
-        if ((x + 1.01 * h - xend) * posneg > 0.0) {
-            h = xend - x;
-            last = true;
-        }
-        ++nstep;
-        double k2x, k2y, k3x, k3y, k4x, k4y, k5x, k5y, k6x, k6y, k7x, k7y, k8x, k8y, k9x, k9y,
-            k10x, k10y;
-        f(x + c2 * h, y0 + h * a21 * k1x, y1 + h * a21 * k1y, k2x, k2y);
-        f(x + c3 * h, y0 + h * (a31 * k1x + a32 * k2x), y1 + h * (a31 * k1y + a32 * k2y), k3x,
-          k3y);
-        f(x + c4 * h, y0 + h * (a41 * k1x + a43 * k3x), y1 + h * (a41 * k1y + a43 * k3y), k4x,
-          k4y);
-        f(x + c5 * h, y0 + h * (a51 * k1x + a53 * k3x + a54 * k4x),
-          y1 + h * (a51 * k1y + a53 * k3y + a54 * k4y), k5x, k5y);
-        f(x + c6 * h, y0 + h * (a61 * k1x + a64 * k4x + a65 * k5x),
-          y1 + h * (a61 * k1y + a64 * k4y + a65 * k5y), k6x, k6y);
-        f(x + c7 * h, y0 + h * (a71 * k1x + a74 * k4x + a75 * k5x + a76 * k6x),
-          y1 + h * (a71 * k1y + a74 * k4y + a75 * k5y + a76 * k6y), k7x, k7y);
-        f(x + c8 * h, y0 + h * (a81 * k1x + a84 * k4x + a85 * k5x + a86 * k6x + a87 * k7x),
-          y1 + h * (a81 * k1y + a84 * k4y + a85 * k5y + a86 * k6y + a87 * k7y), k8x, k8y);
-        f(x + c9 * h,
-          y0 + h * (a91 * k1x + a94 * k4x + a95 * k5x + a96 * k6x + a97 * k7x + a98 * k8x),
-          y1 + h * (a91 * k1y + a94 * k4y + a95 * k5y + a96 * k6y + a97 * k7y + a98 * k8y), k9x,
-          k9y);
-        f(x + c10 * h,
-          y0 + h * (a101 * k1x + a104 * k4x + a105 * k5x + a106 * k6x + a107 * k7x +
-                    a108 * k8x + a109 * k9x),
-          y1 + h * (a101 * k1y + a104 * k4y + a105 * k5y + a106 * k6y + a107 * k7y +
-                    a108 * k8y + a109 * k9y),
-          k10x, k10y);
-        // stage 11 reuses k2, stage 12 reuses k3 (as in Hairer's code)
-        f(x + c11 * h,
-          y0 + h * (a111 * k1x + a114 * k4x + a115 * k5x + a116 * k6x + a117 * k7x +
-                    a118 * k8x + a119 * k9x + a1110 * k10x),
-          y1 + h * (a111 * k1y + a114 * k4y + a115 * k5y + a116 * k6y + a117 * k7y +
-                    a118 * k8y + a119 * k9y + a1110 * k10y),
-          k2x, k2y);
-        const double xph = x + h;
-        f(xph,
-          y0 + h * (a121 * k1x + a124 * k4x + a125 * k5x + a126 * k6x + a127 * k7x +
-                    a128 * k8x + a129 * k9x + a1210 * k10x + a1211 * k2x),
-          y1 + h * (a121 * k1y + a124 * k4y + a125 * k5y + a126 * k6y + a127 * k7y +
-                    a128 * k8y + a129 * k9y + a1210 * k10y + a1211 * k2y),
-          k3x, k3y);
-        const double k4nx = b1 * k1x + b6 * k6x + b7 * k7x + b8 * k8x + b9 * k9x + b10 * k10x +
-                            b11 * k2x + b12 * k3x;
-        const double k4ny = b1 * k1y + b6 * k6y + b7 * k7y + b8 * k8y + b9 * k9y + b10 * k10y +
-                            b11 * k2y + b12 * k3y;
-        const double k5nx = y0 + h * k4nx;
-        const double k5ny = y1 + h * k4ny;
-        // error estimate: blended 5th/3rd order norms
-        const double skx = atol + rtol * fmax(fabs(y0), fabs(k5nx));
-        const double sky = atol + rtol * fmax(fabs(y1), fabs(k5ny));
-        const double e2x = (k4nx - bhh1 * k1x - bhh2 * k9x - bhh3 * k3x) / skx;
-        const double e2y = (k4ny - bhh1 * k1y - bhh2 * k9y - bhh3 * k3y) / sky;
-        const double e5x = (er1 * k1x + er6 * k6x + er7 * k7x + er8 * k8x + er9 * k9x +
-                            er10 * k10x + er11 * k2x + er12 * k3x) / skx;
-        const double e5y = (er1 * k1y + er6 * k6y + er7 * k7y + er8 * k8y + er9 * k9y +
-                            er10 * k10y + er11 * k2y + er12 * k3y) / sky;
-        const double err2 = e2x * e2x + e2y * e2y;
-        double err = e5x * e5x + e5y * e5y;
-        double deno = err + 0.01 * err2;
-        if (deno <= 0.0) deno = 1.0;
-        err = fabs(h) * err * sqrt(1.0 / (2 * deno));
-        const double fac11 = pow(err, expo1);
-        const double fac = fmax(facc2, fmin(facc1, fac11 / safe));
-        double hnew = h / fac;
-        if (err <= 1.0) {
-            facold = fmax(err, 1.0e-4);
-            f(xph, k5nx, k5ny, k1x, k1y);  // FSAL: derivative at the new point
-            y0 = k5nx;
-            y1 = k5ny;
-            x = xph;
-            if (last) break;
-            if (fabs(hnew) > hmax) hnew = posneg * hmax;
-            if (reject) hnew = posneg * fmin(fabs(hnew), fabs(h));
-            reject = false;
-        } else {
-            hnew = h / fmin(facc1, fac11 / safe);
-            reject = true;
-            last = false;
-        }
-        h = hnew;
+    if (s.nstep > nmax) {
+        s.idid = -2;
+        return false;
     }
-    (void)facold;
-    Y0 = y0;
-    Y1 = y1;
-    return SolveResult{idid, static_cast<int>(nstep)};
+    if (0.1 * fabs(s.h) <= fabs(s.x) * uround) {
+        s.idid = -3;
+        return false;
+    }
+    if ((s.x + 1.01 * s.h - s.xend) * s.posneg > 0.0) {
+        s.h = s.xend - s.x;
+        s.last = true;
+    }
+    ++s.nstep;
+    const double x = s.x, h = s.h;
+    // Stage sums in double for both forms: DOP853's coefficients are large
+    // and cancel (|a98| = 43.5), so float sums would cost ~1e-6 relative per
+    // stage, the size of the tolerance itself.  In the float form only the
+    // stored stages, the stored displacement and the velocity evaluation are
+    // float.
+    auto D = [](R v) { return static_cast<double>(v); };
+    const double ax = s.ax, ay = s.ay;
+    const double y0 = ax + D(s.dx), y1 = ay + D(s.dy);  // = s.dx, s.dy exactly for double
+    const double K1x = D(s.k1x), K1y = D(s.k1y);
+    R k2x, k2y, k3x, k3y, k4x, k4y, k5x, k5y, k6x, k6y, k7x, k7y, k8x, k8y, k9x, k9y, k10x, k10y;
+#define NBX_K(n) const double K##n##x = D(k##n##x), K##n##y = D(k##n##y)
+    f(x + c2 * h, y0 + h * a21 * K1x, y1 + h * a21 * K1y, k2x, k2y);
+    NBX_K(2);
+    f(x + c3 * h, y0 + h * (a31 * K1x + a32 * K2x), y1 + h * (a31 * K1y + a32 * K2y), k3x, k3y);
+    NBX_K(3);
+    f(x + c4 * h, y0 + h * (a41 * K1x + a43 * K3x), y1 + h * (a41 * K1y + a43 * K3y), k4x, k4y);
+    NBX_K(4);
+    f(x + c5 * h, y0 + h * (a51 * K1x + a53 * K3x + a54 * K4x),
+      y1 + h * (a51 * K1y + a53 * K3y + a54 * K4y), k5x, k5y);
+    NBX_K(5);
+    f(x + c6 * h, y0 + h * (a61 * K1x + a64 * K4x + a65 * K5x),
+      y1 + h * (a61 * K1y + a64 * K4y + a65 * K5y), k6x, k6y);
+    NBX_K(6);
+    f(x + c7 * h, y0 + h * (a71 * K1x + a74 * K4x + a75 * K5x + a76 * K6x),
+      y1 + h * (a71 * K1y + a74 * K4y + a75 * K5y + a76 * K6y), k7x, k7y);
+    NBX_K(7);
+    f(x + c8 * h, y0 + h * (a81 * K1x + a84 * K4x + a85 * K5x + a86 * K6x + a87 * K7x),
+      y1 + h * (a81 * K1y + a84 * K4y + a85 * K5y + a86 * K6y + a87 * K7y), k8x, k8y);
+    NBX_K(8);
+    f(x + c9 * h,
+      y0 + h * (a91 * K1x + a94 * K4x + a95 * K5x + a96 * K6x + a97 * K7x + a98 * K8x),
+      y1 + h * (a91 * K1y + a94 * K4y + a95 * K5y + a96 * K6y + a97 * K7y + a98 * K8y), k9x,
+      k9y);
+    NBX_K(9);
+    f(x + c10 * h,
+      y0 + h * (a101 * K1x + a104 * K4x + a105 * K5x + a106 * K6x + a107 * K7x + a108 * K8x +
+                a109 * K9x),
+      y1 + h * (a101 * K1y + a104 * K4y + a105 * K5y + a106 * K6y + a107 * K7y + a108 * K8y +
+                a109 * K9y),
+      k10x, k10y);
+    NBX_K(10);
+    // stage 11 reuses k2, stage 12 reuses k3 (as in Hairer's code)
+    f(x + c11 * h,
+      y0 + h * (a111 * K1x + a114 * K4x + a115 * K5x + a116 * K6x + a117 * K7x + a118 * K8x +
+                a119 * K9x + a1110 * K10x),
+      y1 + h * (a111 * K1y + a114 * K4y + a115 * K5y + a116 * K6y + a117 * K7y + a118 * K8y +
+                a119 * K9y + a1110 * K10y),
+      k2x, k2y);
+    const double L2x = D(k2x), L2y = D(k2y);
+    const double xph = x + h;
+    f(xph,
+      y0 + h * (a121 * K1x + a124 * K4x + a125 * K5x + a126 * K6x + a127 * K7x + a128 * K8x +
+                a129 * K9x + a1210 * K10x + a1211 * L2x),
+      y1 + h * (a121 * K1y + a124 * K4y + a125 * K5y + a126 * K6y + a127 * K7y + a128 * K8y +
+                a129 * K9y + a1210 * K10y + a1211 * L2y),
+      k3x, k3y);
+    const double L3x = D(k3x), L3y = D(k3y);
+#undef NBX_K
+    const double k4nx = b1 * K1x + b6 * K6x + b7 * K7x + b8 * K8x + b9 * K9x + b10 * K10x +
+                        b11 * L2x + b12 * L3x;
+    const double k4ny = b1 * K1y + b6 * K6y + b7 * K7y + b8 * K8y + b9 * K9y + b10 * K10y +
+                        b11 * L2y + b12 * L3y;
+    const double pnx = y0 + h * k4nx;
+    const double pny = y1 + h * k4ny;
+    // error estimate: blended 5th/3rd order norms
+    const double skx = atol + rtol * fmax(fabs(y0), fabs(pnx));
+    const double sky = atol + rtol * fmax(fabs(y1), fabs(pny));
+    const double e2x = (k4nx - bhh1 * K1x - bhh2 * K9x - bhh3 * L3x) / skx;
+    const double e2y = (k4ny - bhh1 * K1y - bhh2 * K9y - bhh3 * L3y) / sky;
+    const double e5x = (er1 * K1x + er6 * K6x + er7 * K7x + er8 * K8x + er9 * K9x +
+                        er10 * K10x + er11 * L2x + er12 * L3x) / skx;
+    const double e5y = (er1 * K1y + er6 * K6y + er7 * K7y + er8 * K8y + er9 * K9y +
+                        er10 * K10y + er11 * L2y + er12 * L3y) / sky;
+    const double err2 = e2x * e2x + e2y * e2y;
+    double err = e5x * e5x + e5y * e5y;
+    double deno = err + 0.01 * err2;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (2 * deno));
+    const double fac11 = pow(err, expo1);
+    const double fac = fmax(facc2, fmin(facc1, fac11 / safe));
+    double hnew = h / fac;
+    if (err <= 1.0) {
+        s.facold = fmax(err, 1.0e-4);
+        // new state (exact for double, where the anchor is 0), then the
+        // derivative at the stored point
+        s.dx = static_cast<R>(pnx - ax);
+        s.dy = static_cast<R>(pny - ay);
+        f(xph, ax + D(s.dx), ay + D(s.dy), s.k1x, s.k1y);
+        s.x = xph;
+        if (s.last) {
+            s.idid = 1;
+            return false;
+        }
+        if (fabs(hnew) > s.hmax) hnew = s.posneg * s.hmax;
+        if (s.reject) hnew = s.posneg * fmin(fabs(hnew), fabs(h));
+        s.reject = false;
+    } else {
+        hnew = h / fmin(facc1, fac11 / safe);
+        s.reject = true;
+        s.last = false;
+    }
+    s.h = hnew;
+    return true;
 }
 
 }  // namespace nbx

@@ -176,85 +176,90 @@ __global__ void k_interleave(const double* __restrict__ u, const double* __restr
 // ---------------------------------------------------------------------------
 // Flow-map kernels.
 
-template <class Flow>
-__global__ __launch_bounds__(256) void k_flowmap_grid(Flow f, const double* __restrict__ xs, int nx,
-                                                      const double* __restrict__ ys, int ny,
-                                                      double s0, double s1, double rtol,
-                                                      double atol,
-                                                      const uint8_t* __restrict__ mask,
-                                                      long long nmax, double2* __restrict__ out,
-                                                      int32_t* __restrict__ status,
-                                                      int32_t* __restrict__ nsteps) {
-    const long long p = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-    if (p >= static_cast<long long>(nx) * ny) return;
-    const int i = static_cast<int>(p / ny);
-    const int j = static_cast<int>(p - static_cast<long long>(i) * ny);
+// Particle sources: a tensor grid x[i], y[j] in 'ij' order, or a point list.
+struct GridSource {
+    const double* __restrict__ xs;
+    const double* __restrict__ ys;
+    int ny;
+    __device__ __forceinline__ void pos(int64_t p, double& px, double& py) const {
+        const int64_t i = p / ny;
+        px = xs[i];
+        py = ys[p - i * ny];
+    }
+};
+
+struct PointSource {
+    const double2* __restrict__ pts;
+    __device__ __forceinline__ void pos(int64_t p, double& px, double& py) const {
+        const double2 q = pts[p];
+        px = q.x;
+        py = q.y;
+    }
+};
+
+// One solve per group of G lanes (G = 1 for analytic flows); the group's
+// lanes run identical solver arithmetic and split the spline gathers.
+template <class Flow, class R, int G, class Src>
+__global__ __launch_bounds__(256) void k_flowmap(Flow f, Src src, int64_t P, double s0, double s1,
+                                                 double rtol, double atol,
+                                                 const uint8_t* __restrict__ mask, long long nmax,
+                                                 double2* __restrict__ out,
+                                                 int32_t* __restrict__ status,
+                                                 int32_t* __restrict__ nsteps) {
+    const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const int64_t p = tid / G;
+    if (p >= P) return;  // whole groups leave together
+    const bool lead = (tid & (G - 1)) == 0;
     if (mask != nullptr && mask[p]) {
-        out[p] = make_double2(0.0, 0.0);
-        if (status) status[p] = NBX_ST_MASKED;
-        if (nsteps) nsteps[p] = 0;
+        if (lead) {
+            out[p] = make_double2(0.0, 0.0);
+            if (status) status[p] = NBX_ST_MASKED;
+            if (nsteps) nsteps[p] = 0;
+        }
         return;
     }
-    double a = xs[i], b = ys[j];
-    const SolveResult r = dop853_solve(f, s0, s1, a, b, rtol, atol, nmax);
-    out[p] = make_double2(a, b);
-    if (status) status[p] = r.idid;
-    if (nsteps) nsteps[p] = r.nstep;
-}
-
-template <class Flow>
-__global__ __launch_bounds__(256) void k_flowmap_pts(Flow f, const double2* __restrict__ pts,
-                                                     int64_t npts, double s0, double s1,
-                                                     double rtol, double atol,
-                                                     const uint8_t* __restrict__ mask,
-                                                     long long nmax, double2* __restrict__ out,
-                                                     int32_t* __restrict__ status,
-                                                     int32_t* __restrict__ nsteps) {
-    const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (p >= npts) return;
-    if (mask != nullptr && mask[p]) {
-        out[p] = make_double2(0.0, 0.0);
-        if (status) status[p] = NBX_ST_MASKED;
-        if (nsteps) nsteps[p] = 0;
-        return;
+    double px, py;
+    src.pos(p, px, py);
+    Dop853State<R> s;
+    dop853_init(f, s, s0, s1, px, py, rtol, atol);
+    while (dop853_step(f, s, rtol, atol, nmax)) {
     }
-    double2 q = pts[p];
-    const SolveResult r = dop853_solve(f, s0, s1, q.x, q.y, rtol, atol, nmax);
-    out[p] = q;
-    if (status) status[p] = r.idid;
-    if (nsteps) nsteps[p] = r.nstep;
+    if (lead) {
+        out[p] = make_double2(s.px(), s.py());
+        if (status) status[p] = s.idid;
+        if (nsteps) nsteps[p] = s.nstep;
+    }
 }
 
-template <class Flow>
+template <class Flow, class R>
 __global__ void k_flow_eval(Flow f, const double* __restrict__ t, const double2* __restrict__ pts,
                             int64_t n, double2* __restrict__ out) {
     const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (k >= n) return;
-    double a, b;
+    R a, b;
     f(t[k], pts[k].x, pts[k].y, a, b);
-    out[k] = make_double2(a, b);
+    out[k] = make_double2(static_cast<double>(a), static_cast<double>(b));
 }
 
-// Build the device functor of a flow for one launch.
-int make_analytic(const nbx_flow* fl, const double* params, int np, DoubleGyre* dg,
-                  BickleyJet* bj) {
-    if (fl->kind == NBX_FLOW_DOUBLE_GYRE) {
-        NBX_REQUIRE(np >= 6, "double_gyre needs 6 parameters [dir, A, eps, alpha, omega, psi]");
-        *dg = DoubleGyre{params[0], params[1], params[2], params[3], params[4], params[5]};
-        return 0;
-    }
-    if (fl->kind == NBX_FLOW_BICKLEY_JET) {
-        NBX_REQUIRE(np >= 12, "bickley_jet needs 12 parameters");
-        for (int k = 0; k < 12; ++k) bj->p[k] = params[k];
-        return 0;
-    }
-    set_error("not an analytic flow");
-    return -2;
+// Launch shape per (flow kind, precision): lanes per particle for spline
+// flows (1, 2 or 4) and threads per block.  Set by the autotuner through
+// nbx_set_launch_config; the defaults are the measured best for the
+// gridded-velocity workload.
+struct LaunchCfg {
+    int lanes;
+    int block;
+};
+LaunchCfg g_cfg[3][2] = {{{1, 256}, {1, 256}}, {{1, 256}, {1, 256}}, {{1, 256}, {1, 256}}};
+std::mutex g_cfg_mutex;
+
+LaunchCfg launch_cfg(int kind, int precision) {
+    std::lock_guard<std::mutex> lock(g_cfg_mutex);
+    return g_cfg[kind][precision];
 }
 
 template <class R>
-Spline3<R> make_spline(const nbx_flow* fl, double p0) {
-    Spline3<R> s;
+Spline3<R, 1> make_spline(const nbx_flow* fl, double p0) {
+    Spline3<R, 1> s;
     if constexpr (sizeof(R) == 8) s.C = fl->coef;
     else s.C = fl->coef32;
     for (int d = 0; d < 3; ++d) {
@@ -272,6 +277,78 @@ Spline3<R> make_spline(const nbx_flow* fl, double p0) {
     s.r = fl->r;
     s.p0 = p0;
     return s;
+}
+
+template <int G, class R>
+Spline3<R, G> regroup(const Spline3<R, 1>& a) {
+    Spline3<R, G> s;
+    s.C = a.C;
+    for (int d = 0; d < 3; ++d) {
+        s.lo[d] = a.lo[d];
+        s.hi[d] = a.hi[d];
+        s.dinv[d] = a.dinv[d];
+        s.nm2[d] = a.nm2[d];
+    }
+    s.sx = a.sx;
+    s.st = a.st;
+    s.spherical = a.spherical;
+    s.extrap = a.extrap;
+    s.r = a.r;
+    s.p0 = a.p0;
+    return s;
+}
+
+struct SolveArgs {
+    int64_t P;
+    double s0, s1, rtol, atol;
+    const uint8_t* mask;
+    long long nmax;
+    double2* out;
+    int32_t* status;
+    int32_t* nsteps;
+};
+
+template <class Flow, class R, int G, class Src>
+int launch_solve(const Flow& f, const Src& src, const SolveArgs& a, int block, hipStream_t s) {
+    const int64_t threads = a.P * G;
+    k_flowmap<Flow, R, G, Src><<<blocks_for(threads, block), block, 0, s>>>(
+        f, src, a.P, a.s0, a.s1, a.rtol, a.atol, a.mask, a.nmax, a.out, a.status, a.nsteps);
+    NBX_LAUNCHED("flowmap");
+    return 0;
+}
+
+template <class R, class Src>
+int dispatch_solve(const nbx_flow* fl, const double* params, int np, const Src& src,
+                   const SolveArgs& a, int precision, hipStream_t s) {
+    const LaunchCfg cfg = launch_cfg(fl->kind, precision);
+    int block = cfg.block;
+    if (block != 64 && block != 128 && block != 256) block = 256;
+    if (fl->kind == NBX_FLOW_SPLINE3) {
+        const Spline3<R, 1> sp = make_spline<R>(fl, params[0]);
+        if (cfg.lanes == 4) return launch_solve<Spline3<R, 4>, R, 4>(regroup<4>(sp), src, a, block, s);
+        if (cfg.lanes == 2) return launch_solve<Spline3<R, 2>, R, 2>(regroup<2>(sp), src, a, block, s);
+        return launch_solve<Spline3<R, 1>, R, 1>(sp, src, a, block, s);
+    }
+    if (fl->kind == NBX_FLOW_DOUBLE_GYRE) {
+        NBX_REQUIRE(np >= 6, "double_gyre needs 6 parameters [dir, A, eps, alpha, omega, psi]");
+        const DoubleGyre<R> dg{params[0], params[1], params[2], params[3], params[4], params[5]};
+        return launch_solve<DoubleGyre<R>, R, 1>(dg, src, a, block, s);
+    }
+    if (fl->kind == NBX_FLOW_BICKLEY_JET) {
+        NBX_REQUIRE(np >= 12, "bickley_jet needs 12 parameters");
+        BickleyJet<R> bj;
+        for (int k = 0; k < 12; ++k) bj.p[k] = params[k];
+        return launch_solve<BickleyJet<R>, R, 1>(bj, src, a, block, s);
+    }
+    set_error("unknown flow kind");
+    return -2;
+}
+
+template <class Src>
+int solve(const nbx_flow* fl, const double* params, int np, const Src& src, const SolveArgs& a,
+          int precision, hipStream_t s) {
+    if (precision == NBX_FP32_DISP) return dispatch_solve<float>(fl, params, np, src, a, precision, s);
+    return dispatch_solve<double>(fl, params, np, src, a, precision, s);
 }
 
 }  // namespace nbx
@@ -372,16 +449,29 @@ int nbx_flow_eval(const nbx_flow* fl, const double* params, int np, const double
     auto* O = reinterpret_cast<double2*>(out);
     const unsigned nb = blocks_for(n, 256);
     if (fl->kind == NBX_FLOW_SPLINE3) {
-        k_flow_eval<<<nb, 256, 0, s>>>(make_spline<double>(fl, params[0]), t, P, n, O);
+        k_flow_eval<Spline3<double, 1>, double><<<nb, 256, 0, s>>>(
+            make_spline<double>(fl, params[0]), t, P, n, O);
+    } else if (fl->kind == NBX_FLOW_DOUBLE_GYRE) {
+        NBX_REQUIRE(np >= 6, "double_gyre needs 6 parameters");
+        const DoubleGyre<double> dg{params[0], params[1], params[2], params[3], params[4], params[5]};
+        k_flow_eval<DoubleGyre<double>, double><<<nb, 256, 0, s>>>(dg, t, P, n, O);
     } else {
-        DoubleGyre dg{};
-        BickleyJet bj{};
-        int rc = make_analytic(fl, params, np, &dg, &bj);
-        if (rc) return rc;
-        if (fl->kind == NBX_FLOW_DOUBLE_GYRE) k_flow_eval<<<nb, 256, 0, s>>>(dg, t, P, n, O);
-        else k_flow_eval<<<nb, 256, 0, s>>>(bj, t, P, n, O);
+        NBX_REQUIRE(np >= 12, "bickley_jet needs 12 parameters");
+        BickleyJet<double> bj;
+        for (int k = 0; k < 12; ++k) bj.p[k] = params[k];
+        k_flow_eval<BickleyJet<double>, double><<<nb, 256, 0, s>>>(bj, t, P, n, O);
     }
     NBX_LAUNCHED("flow_eval");
+    return 0;
+}
+
+static int check_solve_args(const nbx_flow* fl, const double* params, int np, double rtol,
+                            double atol, int64_t max_steps, int precision) {
+    NBX_REQUIRE(fl != nullptr, "flow handle is NULL");
+    NBX_REQUIRE(params != nullptr && np >= 1, "params must hold at least the direction");
+    NBX_REQUIRE(rtol > 0 && atol >= 0, "tolerances must be positive");
+    NBX_REQUIRE(max_steps >= 1, "max_steps must be >= 1");
+    NBX_REQUIRE(precision == NBX_FP64 || precision == NBX_FP32_DISP, "unknown precision");
     return 0;
 }
 
@@ -389,73 +479,51 @@ int nbx_flowmap_grid_2d(const nbx_flow* fl, double t0, double T, const double* x
                         const double* y, int ny, const double* params, int np, double rtol,
                         double atol, const uint8_t* mask, int64_t max_steps, int precision,
                         double* out, int32_t* status, int32_t* nsteps, nbx_stream stream) {
-    NBX_REQUIRE(fl != nullptr, "flow handle is NULL");
-    NBX_REQUIRE(params != nullptr && np >= 1, "params must hold at least the direction");
+    if (int rc = check_solve_args(fl, params, np, rtol, atol, max_steps, precision)) return rc;
     NBX_REQUIRE(nx >= 0 && ny >= 0, "negative grid size");
-    NBX_REQUIRE(rtol > 0 && atol >= 0, "tolerances must be positive");
-    NBX_REQUIRE(max_steps >= 1, "max_steps must be >= 1");
-    NBX_REQUIRE(precision == NBX_FP64, "only the FP64 flow map is available in this entry point");
     const int64_t P = static_cast<int64_t>(nx) * ny;
     if (P == 0) return 0;
     NBX_REQUIRE(x != nullptr && y != nullptr && out != nullptr, "NULL array");
-    hipStream_t s = as_stream(stream);
-    const double s0 = params[0] * t0, s1 = params[0] * (t0 + T);
-    auto* O = reinterpret_cast<double2*>(out);
-    const unsigned nb = blocks_for(P, 256);
-    const long long nmax = max_steps;
-    if (fl->kind == NBX_FLOW_SPLINE3) {
-        k_flowmap_grid<<<nb, 256, 0, s>>>(make_spline<double>(fl, params[0]), x, nx, y, ny, s0,
-                                          s1, rtol, atol, mask, nmax, O, status, nsteps);
-    } else {
-        DoubleGyre dg{};
-        BickleyJet bj{};
-        int rc = make_analytic(fl, params, np, &dg, &bj);
-        if (rc) return rc;
-        if (fl->kind == NBX_FLOW_DOUBLE_GYRE)
-            k_flowmap_grid<<<nb, 256, 0, s>>>(dg, x, nx, y, ny, s0, s1, rtol, atol, mask, nmax, O,
-                                              status, nsteps);
-        else
-            k_flowmap_grid<<<nb, 256, 0, s>>>(bj, x, nx, y, ny, s0, s1, rtol, atol, mask, nmax, O,
-                                              status, nsteps);
-    }
-    NBX_LAUNCHED("flowmap_grid");
-    return 0;
+    SolveArgs a{P, params[0] * t0, params[0] * (t0 + T), rtol, atol, mask, max_steps,
+                reinterpret_cast<double2*>(out), status, nsteps};
+    return solve(fl, params, np, GridSource{x, y, ny}, a, precision, as_stream(stream));
 }
 
 int nbx_flowmap_pts(const nbx_flow* fl, double t0, double T, const double* pts, int64_t npts,
                     const double* params, int np, double rtol, double atol, const uint8_t* mask,
                     int64_t max_steps, int precision, double* out, int32_t* status,
                     int32_t* nsteps, nbx_stream stream) {
-    NBX_REQUIRE(fl != nullptr, "flow handle is NULL");
-    NBX_REQUIRE(params != nullptr && np >= 1, "params must hold at least the direction");
+    if (int rc = check_solve_args(fl, params, np, rtol, atol, max_steps, precision)) return rc;
     NBX_REQUIRE(npts >= 0, "negative point count");
-    NBX_REQUIRE(rtol > 0 && atol >= 0, "tolerances must be positive");
-    NBX_REQUIRE(max_steps >= 1, "max_steps must be >= 1");
-    NBX_REQUIRE(precision == NBX_FP64, "only the FP64 flow map is available in this entry point");
     if (npts == 0) return 0;
     NBX_REQUIRE(pts != nullptr && out != nullptr, "NULL array");
-    hipStream_t s = as_stream(stream);
-    const double s0 = params[0] * t0, s1 = params[0] * (t0 + T);
-    const auto* Pp = reinterpret_cast<const double2*>(pts);
-    auto* O = reinterpret_cast<double2*>(out);
-    const unsigned nb = blocks_for(npts, 256);
-    const long long nmax = max_steps;
-    if (fl->kind == NBX_FLOW_SPLINE3) {
-        k_flowmap_pts<<<nb, 256, 0, s>>>(make_spline<double>(fl, params[0]), Pp, npts, s0, s1,
-                                         rtol, atol, mask, nmax, O, status, nsteps);
-    } else {
-        DoubleGyre dg{};
-        BickleyJet bj{};
-        int rc = make_analytic(fl, params, np, &dg, &bj);
-        if (rc) return rc;
-        if (fl->kind == NBX_FLOW_DOUBLE_GYRE)
-            k_flowmap_pts<<<nb, 256, 0, s>>>(dg, Pp, npts, s0, s1, rtol, atol, mask, nmax, O,
-                                             status, nsteps);
-        else
-            k_flowmap_pts<<<nb, 256, 0, s>>>(bj, Pp, npts, s0, s1, rtol, atol, mask, nmax, O,
-                                             status, nsteps);
-    }
-    NBX_LAUNCHED("flowmap_pts");
+    SolveArgs a{npts, params[0] * t0, params[0] * (t0 + T), rtol, atol, mask, max_steps,
+                reinterpret_cast<double2*>(out), status, nsteps};
+    return solve(fl, params, np, PointSource{reinterpret_cast<const double2*>(pts)}, a, precision,
+                 as_stream(stream));
+}
+
+int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int block_threads) {
+    NBX_REQUIRE(kind >= 0 && kind <= 2, "unknown flow kind");
+    NBX_REQUIRE(precision == NBX_FP64 || precision == NBX_FP32_DISP, "unknown precision");
+    NBX_REQUIRE(lanes_per_particle == 1 || lanes_per_particle == 2 || lanes_per_particle == 4,
+                "lanes_per_particle must be 1, 2 or 4");
+    NBX_REQUIRE(block_threads == 64 || block_threads == 128 || block_threads == 256,
+                "block_threads must be 64, 128 or 256");
+    NBX_REQUIRE(kind == NBX_FLOW_SPLINE3 || lanes_per_particle == 1,
+                "analytic flows run one lane per particle");
+    std::lock_guard<std::mutex> lock(g_cfg_mutex);
+    g_cfg[kind][precision] = LaunchCfg{lanes_per_particle, block_threads};
+    return 0;
+}
+
+int nbx_get_launch_config(int kind, int precision, int* lanes_per_particle, int* block_threads) {
+    NBX_REQUIRE(kind >= 0 && kind <= 2, "unknown flow kind");
+    NBX_REQUIRE(precision == NBX_FP64 || precision == NBX_FP32_DISP, "unknown precision");
+    NBX_REQUIRE(lanes_per_particle != nullptr && block_threads != nullptr, "NULL output");
+    const LaunchCfg c = launch_cfg(kind, precision);
+    *lanes_per_particle = c.lanes;
+    *block_threads = c.block;
     return 0;
 }
 

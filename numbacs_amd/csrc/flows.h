@@ -3,6 +3,9 @@
 // integrator; it evaluates dy = p0 * v(p0 * t, y), the sign-flipped
 // right-hand side convention of the reference (params[0] = integration
 // direction, flows.py:1146-1158, docs userguide.rst:216-227).
+//
+// Call form: f(t, px, py, f0, f1) with time and position in double and the
+// velocity returned in the stage precision R of the integrator.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,42 +17,85 @@ namespace nbx {
 constexpr double kPi = 3.141592653589793;
 
 // Double gyre, flows.py:1146-1158.  p = [dir, A, eps, alpha, omega, psi].
+// The float form evaluates the same expression in float.
+template <class R>
 struct DoubleGyre {
     double p0, A, eps, alpha, omega, psi;
 
-    __device__ __forceinline__ void operator()(double t, double y0, double y1, double& f0,
-                                               double& f1) const {
-        const double tt = p0 * t;
-        const double a = eps * sin(omega * tt + psi);
-        const double b = 1 - 2 * a;
-        const double f = a * (y0 * y0) + b * y0;
-        const double df = 2 * a * y0 + b;
-        double sf, cf, sy, cy;
-        sincos(kPi * f, &sf, &cf);
-        sincos(kPi * y1, &sy, &cy);
-        f0 = p0 * (-kPi * A * sf * cy - alpha * y0);
-        f1 = p0 * (kPi * A * cf * sy * df - alpha * y1);
+    __device__ __forceinline__ void operator()(double t, double y0d, double y1d, R& f0,
+                                               R& f1) const {
+        if constexpr (sizeof(R) == sizeof(double)) {
+            const double y0 = y0d, y1 = y1d;
+            const double tt = p0 * t;
+            const double a = eps * sin(omega * tt + psi);
+            const double b = 1 - 2 * a;
+            const double f = a * (y0 * y0) + b * y0;
+            const double df = 2 * a * y0 + b;
+            double sf, cf, sy, cy;
+            sincos(kPi * f, &sf, &cf);
+            sincos(kPi * y1, &sy, &cy);
+            f0 = p0 * (-kPi * A * sf * cy - alpha * y0);
+            f1 = p0 * (kPi * A * cf * sy * df - alpha * y1);
+        } else {
+            // time enters through sin(omega*tt + psi): keep that argument in
+            // double, the rest in float
+            const float y0 = static_cast<float>(y0d), y1 = static_cast<float>(y1d);
+            const double tt = p0 * t;
+            const float a = static_cast<float>(eps * sin(omega * tt + psi));
+            const float b = 1.0f - 2.0f * a;
+            const float f = a * (y0 * y0) + b * y0;
+            const float df = 2.0f * a * y0 + b;
+            const float pi = static_cast<float>(kPi);
+            float sf, cf, sy, cy;
+            sincosf(pi * f, &sf, &cf);
+            sincosf(pi * y1, &sy, &cy);
+            const float q0 = static_cast<float>(p0), qA = static_cast<float>(A),
+                        qal = static_cast<float>(alpha);
+            f0 = q0 * (-pi * qA * sf * cy - qal * y0);
+            f1 = q0 * (pi * qA * cf * sy * df - qal * y1);
+        }
     }
 };
 
 // Bickley jet with three travelling modes, flows.py:1182-1213.
 // p = [dir, U0, L, A1, A2, A3, k1, k2, k3, c1, c2, c3].
+template <class R>
 struct BickleyJet {
     double p[12];
 
-    __device__ __forceinline__ void operator()(double t, double y0, double y1, double& f0,
-                                               double& f1) const {
+    __device__ __forceinline__ void operator()(double t, double y0, double y1, R& f0,
+                                               R& f1) const {
         const double tt = p[0] * t;
-        const double Y = y1 / p[2];
-        const double ch = cosh(Y);
-        const double sech2 = 1 / (ch * ch);
-        double s1, c1, s2, c2, s3, c3;
-        sincos(p[6] * (y0 - p[9] * tt), &s1, &c1);
-        sincos(p[7] * (y0 - p[10] * tt), &s2, &c2);
-        sincos(p[8] * (y0 - p[11] * tt), &s3, &c3);
-        f0 = p[0] * (p[1] * sech2 + 2 * p[1] * tanh(Y) * sech2 * (p[3] * c1 + p[4] * c2 + p[5] * c3));
-        f1 = -p[0] * (p[1] * p[2] * sech2 *
-                      (p[3] * p[6] * s1 + p[4] * p[7] * s2 + p[5] * p[8] * s3));
+        if constexpr (sizeof(R) == sizeof(double)) {
+            const double Y = y1 / p[2];
+            const double ch = cosh(Y);
+            const double sech2 = 1 / (ch * ch);
+            double s1, c1, s2, c2, s3, c3;
+            sincos(p[6] * (y0 - p[9] * tt), &s1, &c1);
+            sincos(p[7] * (y0 - p[10] * tt), &s2, &c2);
+            sincos(p[8] * (y0 - p[11] * tt), &s3, &c3);
+            f0 = p[0] * (p[1] * sech2 +
+                         2 * p[1] * tanh(Y) * sech2 * (p[3] * c1 + p[4] * c2 + p[5] * c3));
+            f1 = -p[0] * (p[1] * p[2] * sech2 *
+                          (p[3] * p[6] * s1 + p[4] * p[7] * s2 + p[5] * p[8] * s3));
+        } else {
+            // phase arguments k (x - c t) reduced in double, the rest in float
+            const float Y = static_cast<float>(y1 / p[2]);
+            const float ch = coshf(Y);
+            const float sech2 = 1.0f / (ch * ch);
+            float s1, c1, s2, c2, s3, c3;
+            sincosf(static_cast<float>(fmod(p[6] * (y0 - p[9] * tt), 2 * kPi)), &s1, &c1);
+            sincosf(static_cast<float>(fmod(p[7] * (y0 - p[10] * tt), 2 * kPi)), &s2, &c2);
+            sincosf(static_cast<float>(fmod(p[8] * (y0 - p[11] * tt), 2 * kPi)), &s3, &c3);
+            const float q0 = static_cast<float>(p[0]), U0 = static_cast<float>(p[1]),
+                        L = static_cast<float>(p[2]);
+            const float A1 = static_cast<float>(p[3]), A2 = static_cast<float>(p[4]),
+                        A3 = static_cast<float>(p[5]);
+            const float K1 = static_cast<float>(p[6]), K2 = static_cast<float>(p[7]),
+                        K3 = static_cast<float>(p[8]);
+            f0 = q0 * (U0 * sech2 + 2.0f * U0 * tanhf(Y) * sech2 * (A1 * c1 + A2 * c2 + A3 * c3));
+            f1 = -q0 * (U0 * L * sech2 * (A1 * K1 * s1 + A2 * K2 * s2 + A3 * K3 * s3));
+        }
     }
 };
 
@@ -72,6 +118,12 @@ __device__ __forceinline__ void bspline_dweights(R s, R dw[4]) {
     dw[3] = s2 / R(2);
 }
 
+// w[i] for a lane-dependent i without dynamic register indexing
+template <class R>
+__device__ __forceinline__ R pick4(const R w[4], int i) {
+    return i == 0 ? w[0] : (i == 1 ? w[1] : (i == 2 ? w[2] : w[3]));
+}
+
 // Python's float modulo (result takes the sign of the divisor).
 __device__ __forceinline__ double py_mod(double a, double m) {
     double r = fmod(a, m);
@@ -89,16 +141,53 @@ template <class R> struct CoefPair;
 template <> struct CoefPair<double> { using type = double2; };
 template <> struct CoefPair<float> { using type = float2; };
 
+// Sum of a value over the G lanes of an aligned lane group, with a butterfly
+// of DPP quad permutations, so every lane of the group ends with the same
+// bits (the additions are commutative pairwise, and each level combines the
+// same two partial sums in every lane).
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+    if constexpr (G >= 2)
+        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+    if constexpr (G >= 4)
+        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+    auto xchg = [](double a, int ctrl) {
+        const long long b = __double_as_longlong(a);
+        int lo = static_cast<int>(b & 0xffffffffll), hi = static_cast<int>(b >> 32);
+        if (ctrl == 0xB1) {
+            lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);
+            hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
+        } else {
+            lo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false);
+            hi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false);
+        }
+        return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                    static_cast<unsigned int>(lo));
+    };
+    if constexpr (G >= 2) v += xchg(v, 0xB1);
+    if constexpr (G >= 4) v += xchg(v, 0x4E);
+    return v;
+}
+
 // Tricubic natural B-spline velocity on a uniform (t, x, y) grid (the
 // reference's get_flow_2D over interpolation.eval_spline, flows.py:121-258).
-// C holds (nt+2, nx+2, ny+2) (u, v) pairs, y fastest.  R is the storage and
-// evaluation precision of the coefficients (double for the plain path).
+// C holds (nt+2, nx+2, ny+2) (u, v) pairs, y fastest.  R is the coefficient
+// and evaluation precision (double for the plain path, float for the
+// FP32-displacement form).  G lanes (1, 2 or 4, aligned in the wavefront)
+// share one evaluation: lane g of the group gathers the t-slices a with
+// a % G == g (16 pairs each for G = 4) and the partial sums are combined
+// with group_sum, which multiplies the loads in flight per particle by G.
 //
 // Out-of-box queries (parity unpinned: the reference's dependency is not
 // available): extrap 0/2 clamp the query into the box per axis (value of the
 // nearest box point), extrap 1 adds the gradient at that point times the
 // outward offset.
-template <class R>
+template <class R, int G = 1>
 struct Spline3 {
     using P = typename CoefPair<R>::type;
     const P* __restrict__ C;
@@ -118,9 +207,9 @@ struct Spline3 {
         s = static_cast<R>(u - k);
     }
 
-    // (u, v) at an in-box point; with grad != nullptr also d/dt, d/dx, d/dy.
-    __device__ __forceinline__ void eval_in(double qt, double qx, double qy, R& su, R& sv,
-                                            R (*grad)[2]) const {
+    // (u, v) at an in-box point
+    __device__ __forceinline__ void eval_in(double qt, double qx, double qy, R& su,
+                                            R& sv) const {
         int it, ix, iy;
         R s0, s1, s2;
         cell(0, qt, it, s0);
@@ -131,58 +220,82 @@ struct Spline3 {
         bspline_weights(s1, wx);
         bspline_weights(s2, wy);
         const P* base = C + (static_cast<long>(it) * st + static_cast<long>(ix) * sx + iy);
+        const int g = G > 1 ? static_cast<int>(__lane_id() & (G - 1)) : 0;
         R au = 0, av = 0;
-        if (grad == nullptr) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
+        for (int aa = 0; aa < 4 / G; ++aa) {
+            const int a = aa * G + g;
+            const R wa = G == 1 ? wt[aa] : pick4(wt, a);
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const P* row = base + (a * st + b * sx);
-                    const P c0 = row[0], c1 = row[1], c2 = row[2], c3 = row[3];
-                    R ru = wy[0] * c0.x;
-                    R rv = wy[0] * c0.y;
-                    ru += wy[1] * c1.x;
-                    rv += wy[1] * c1.y;
-                    ru += wy[2] * c2.x;
-                    rv += wy[2] * c2.y;
-                    ru += wy[3] * c3.x;
-                    rv += wy[3] * c3.y;
-                    const R wab = wt[a] * wx[b];
-                    au += wab * ru;
-                    av += wab * rv;
+            for (int b = 0; b < 4; ++b) {
+                const P* row = base + (a * st + b * sx);
+                const P c0 = row[0], c1 = row[1], c2 = row[2], c3 = row[3];
+                R ru = wy[0] * c0.x;
+                R rv = wy[0] * c0.y;
+                ru += wy[1] * c1.x;
+                rv += wy[1] * c1.y;
+                ru += wy[2] * c2.x;
+                rv += wy[2] * c2.y;
+                ru += wy[3] * c3.x;
+                rv += wy[3] * c3.y;
+                const R wab = wa * wx[b];
+                au += wab * ru;
+                av += wab * rv;
+            }
+        }
+        su = group_sum<G>(au);
+        sv = group_sum<G>(av);
+    }
+
+    // value and gradient (d/dt, d/dx, d/dy) at an in-box point; every lane
+    // of the group evaluates the whole stencil (rare: out-of-box queries
+    // with linear extrapolation only)
+    __device__ __forceinline__ void eval_grad(double qt, double qx, double qy, R& su, R& sv,
+                              R g[3][2]) const {
+        int it, ix, iy;
+        R s0, s1, s2;
+        cell(0, qt, it, s0);
+        cell(1, qx, ix, s1);
+        cell(2, qy, iy, s2);
+        R wt[4], wx[4], wy[4], dt[4], dx[4], dy[4];
+        bspline_weights(s0, wt);
+        bspline_weights(s1, wx);
+        bspline_weights(s2, wy);
+        bspline_dweights(s0, dt);
+        bspline_dweights(s1, dx);
+        bspline_dweights(s2, dy);
+        const P* base = C + (static_cast<long>(it) * st + static_cast<long>(ix) * sx + iy);
+        R au = 0, av = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[d][0] = g[d][1] = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const P* row = base + (a * st + b * sx);
+                R ru = 0, rv = 0, du = 0, dv = 0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const P cc = row[c];
+                    ru += wy[c] * cc.x;
+                    rv += wy[c] * cc.y;
+                    du += dy[c] * cc.x;
+                    dv += dy[c] * cc.y;
                 }
+                au += wt[a] * wx[b] * ru;
+                av += wt[a] * wx[b] * rv;
+                g[0][0] += dt[a] * wx[b] * ru;
+                g[0][1] += dt[a] * wx[b] * rv;
+                g[1][0] += wt[a] * dx[b] * ru;
+                g[1][1] += wt[a] * dx[b] * rv;
+                g[2][0] += wt[a] * wx[b] * du;
+                g[2][1] += wt[a] * wx[b] * dv;
             }
-        } else {
-            R dt[4], dx[4], dy[4];
-            bspline_dweights(s0, dt);
-            bspline_dweights(s1, dx);
-            bspline_dweights(s2, dy);
-            R g[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-            for (int a = 0; a < 4; ++a) {
-                for (int b = 0; b < 4; ++b) {
-                    const P* row = base + (a * st + b * sx);
-                    R ru = 0, rv = 0, du = 0, dv = 0;
-                    for (int c = 0; c < 4; ++c) {
-                        const P cc = row[c];
-                        ru += wy[c] * cc.x;
-                        rv += wy[c] * cc.y;
-                        du += dy[c] * cc.x;
-                        dv += dy[c] * cc.y;
-                    }
-                    au += wt[a] * wx[b] * ru;
-                    av += wt[a] * wx[b] * rv;
-                    g[0][0] += dt[a] * wx[b] * ru;
-                    g[0][1] += dt[a] * wx[b] * rv;
-                    g[1][0] += wt[a] * dx[b] * ru;
-                    g[1][1] += wt[a] * dx[b] * rv;
-                    g[2][0] += wt[a] * wx[b] * du;
-                    g[2][1] += wt[a] * wx[b] * dv;
-                }
-            }
-            for (int d = 0; d < 3; ++d) {
-                grad[d][0] = g[d][0] * static_cast<R>(dinv[d]);
-                grad[d][1] = g[d][1] * static_cast<R>(dinv[d]);
-            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            g[d][0] *= static_cast<R>(dinv[d]);
+            g[d][1] *= static_cast<R>(dinv[d]);
         }
         su = au;
         sv = av;
@@ -194,10 +307,10 @@ struct Spline3 {
         const double cy = fmin(fmax(qy, lo[2]), hi[2]);
         const bool inside = (ct == qt) & (cx == qx) & (cy == qy);
         if (inside || extrap != 1) {
-            eval_in(ct, cx, cy, su, sv, nullptr);
+            eval_in(ct, cx, cy, su, sv);
         } else {
             R g[3][2];
-            eval_in(ct, cx, cy, su, sv, g);
+            eval_grad(ct, cx, cy, su, sv, g);
             su += g[0][0] * static_cast<R>(qt - ct) + g[1][0] * static_cast<R>(qx - cx) +
                   g[2][0] * static_cast<R>(qy - cy);
             sv += g[0][1] * static_cast<R>(qt - ct) + g[1][1] * static_cast<R>(qx - cx) +
@@ -205,19 +318,20 @@ struct Spline3 {
         }
     }
 
-    __device__ __forceinline__ void operator()(double t, double y0, double y1, double& f0,
-                                               double& f1) const {
+    __device__ __forceinline__ void operator()(double t, double y0, double y1, R& f0,
+                                               R& f1) const {
         const double tt = p0 * t;
         R su, sv;
         if (spherical == 0) {
             eval(tt, y0, y1, su, sv);
-            f0 = p0 * static_cast<double>(su);
-            f1 = p0 * static_cast<double>(sv);
+            f0 = static_cast<R>(p0 * static_cast<double>(su));
+            f1 = static_cast<R>(p0 * static_cast<double>(sv));
         } else {
             const double xx = spherical == 1 ? py_mod(y0 - 180, 360) - 180 : py_mod(y0, 360);
             eval(tt, xx, y1, su, sv);
-            f0 = ((p0 * static_cast<double>(su)) * 180) / (kPi * r * cos(y1 * kPi / 180));
-            f1 = ((p0 * static_cast<double>(sv)) * 180) / (kPi * r);
+            f0 = static_cast<R>(((p0 * static_cast<double>(su)) * 180) /
+                                (kPi * r * cos(y1 * kPi / 180)));
+            f1 = static_cast<R>(((p0 * static_cast<double>(sv)) * 180) / (kPi * r));
         }
     }
 };
