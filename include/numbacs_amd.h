@@ -97,12 +97,58 @@ int nbx_flowmap_pts(const nbx_flow* flow, double t0, double T, const double* pts
 
 /* Launch shape of the flow-map kernels for one (flow kind, precision):
  * lanes_per_particle (1, 2 or 4; spline flows only: the lanes of a group
- * split each tricubic gather) and threads per block (64, 128 or 256).
+ * split each tricubic gather), threads per block (64, 128 or 256), and
+ * persistent_blocks_per_cu (0: one lane group per particle in one launch;
+ * n > 0: a persistent grid of n blocks per compute unit pulling particles
+ * from a device work queue; grid flow maps and batches only).
  * Process-wide; set by the autotuner (numbacs_amd.autotune).  No reference
  * counterpart (the reference's parallel loop has no tunables,
  * integration.py:166). */
-int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int block_threads);
-int nbx_get_launch_config(int kind, int precision, int* lanes_per_particle, int* block_threads);
+int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int block_threads,
+                          int persistent_blocks_per_cu);
+int nbx_get_launch_config(int kind, int precision, int* lanes_per_particle, int* block_threads,
+                          int* persistent_blocks_per_cu);
+
+/* --- batched FTLE time series ------------------------------------------ */
+
+/* One flow-map request of a batch: window [t0, t0+T], particle axes x (nx),
+ * y (ny), outputs as for nbx_flowmap_grid_2d; with ftle != NULL the batch
+ * also computes that request's FTLE field (grid spacing dx, dy), from the
+ * closed-form eigenvalue (ftle_grid_2D) or, with eigvec_max != NULL, from
+ * the eigen-solver together with eigvec_max (C_eig_2D + ftle_from_eig). */
+typedef struct nbx_request {
+    double t0, T;
+    const double* x;
+    int nx;
+    const double* y;
+    int ny;
+    const uint8_t* mask;   /* nullable */
+    double* flowmap;       /* (nx, ny, 2) */
+    int32_t* status;       /* nullable */
+    int32_t* nsteps;       /* nullable */
+    double* ftle;          /* nullable, (nx, ny) */
+    double* eigvec_max;    /* nullable, (nx, ny, 2) */
+    double dx, dy;
+} nbx_request;
+
+typedef struct nbx_batch nbx_batch;
+
+/* Build a batch over one flow (all requests share params, tolerances and
+ * precision).  Uploads the request table; the output buffers must outlive
+ * every run.  No reference counterpart: the reference loops over t0 in
+ * Python (examples/time_series/plot_dg_time_series.py:101-185). */
+int nbx_batch_create(const nbx_flow* flow, const nbx_request* reqs_host, int nreq,
+                     const double* params_host, int nparams, double rtol, double atol,
+                     int64_t max_steps, int precision, nbx_batch** out);
+int64_t nbx_batch_particles(const nbx_batch* batch);
+/* Enqueue the whole batch: one persistent work-queue flow-map launch over
+ * every request, then one batched FTLE launch. */
+int nbx_batch_run(nbx_batch* batch, nbx_stream stream);
+/* Capture nbx_batch_run into a HIP graph (stream must not be the default
+ * stream) and replay it with nbx_batch_run_graph. */
+int nbx_batch_capture(nbx_batch* batch, nbx_stream stream);
+int nbx_batch_run_graph(nbx_batch* batch, nbx_stream stream);
+void nbx_batch_destroy(nbx_batch* batch);
 
 /* --- FTLE and Cauchy-Green eigen --------------------------------------- */
 
@@ -153,6 +199,41 @@ int nbx_ridges(const double* f, const double* eigvec_max, const double* x, int n
  * Synchronises the stream. */
 int nbx_label8(const uint8_t* b, int nx, int ny, int32_t* labels, int32_t* nlabels_host,
                nbx_stream stream);
+
+/* --- flow-map surrogate (BF16 MFMA MLP) --------------------------------- */
+/* No reference counterpart: kernels of the learned flow-map surrogate that
+ * the FTLE path may use with integrator fallback.  bf16 tensors are raw
+ * 16-bit patterns (uint16_t); GEMM sizes must be multiples of 64 (M, N) and
+ * 32 (K).  mode 0: C16 = bf16(gelu(A B + bias)) and C32 = A B + bias (C32
+ * may be NULL), A (M x K), B (K x N), both row-major; mode 1: C16 =
+ * bf16(bf16(A B^T) * gelu'(pre)), B stored (N x K), pre (M x N) fp32;
+ * mode 2: C32 = A^T B, A stored (K x M), B (K x N); mode 3: C32 = A B. */
+int nbx_mlp_gemm(int mode, const uint16_t* A, const uint16_t* B, int M, int N, int K,
+                 float* C32, uint16_t* C16, const float* bias, const float* pre,
+                 nbx_stream stream);
+/* h0 (n x 2*half) = bf16([sin z, cos z]), z = 2 pi ((inp - shift)/scale) Bf,
+ * inp (n x 4) fp32, Bf (4 x half) fp32. */
+int nbx_mlp_features(const float* inp, int64_t n, const float* shift, const float* scale,
+                     const float* Bf, int half, uint16_t* h0, nbx_stream stream);
+/* y (n x 2) = h (n x width) W (width x 2) + b, fp32 accumulation. */
+int nbx_mlp_out_forward(const uint16_t* h, int64_t n, int width, const uint16_t* W,
+                        const float* b, float* y, nbx_stream stream);
+/* dy = (y - target)/n, *loss = 0.5 * mean |y - target|^2 (device scalar). */
+int nbx_mlp_loss(const float* y, const float* target, int64_t n, float* dy, float* loss,
+                 nbx_stream stream);
+/* Output layer backward: gW (width x 2), gb (2) from h and dy; da (n x width)
+ * = bf16(bf16(dy W^T) * gelu'(pre)). */
+int nbx_mlp_out_backward(const float* dy, const uint16_t* h, const float* pre, int64_t n,
+                         int width, const uint16_t* W, float* gW, float* gb, uint16_t* da,
+                         nbx_stream stream);
+/* gb (width) = column sums of da (n x width). */
+int nbx_mlp_colsum(const uint16_t* da, int64_t n, int width, float* gb, nbx_stream stream);
+/* Adam (bias-corrected, decoupled weight decay) on fp32 master weights w;
+ * w16 (nullable) receives the bf16 copy. */
+int nbx_mlp_adam(float* w, const float* g, float* m, float* v, uint16_t* w16, int64_t n,
+                 float lr, float b1, float b2, float eps, int step, float weight_decay,
+                 nbx_stream stream);
+int nbx_mlp_to_bf16(const float* x, int64_t n, uint16_t* y, nbx_stream stream);
 
 #ifdef __cplusplus
 }

@@ -40,8 +40,24 @@ SIGNATURES = {
                                    _c_d, _c_p, _c_i64, _c_i, _c_p, _c_p, _c_p, _c_p]),
     "nbx_flowmap_pts": (_c_i, [_c_p, _c_d, _c_d, _c_p, _c_i64, _c_p, _c_i, _c_d, _c_d, _c_p,
                                _c_i64, _c_i, _c_p, _c_p, _c_p, _c_p]),
-    "nbx_set_launch_config": (_c_i, [_c_i, _c_i, _c_i, _c_i]),
-    "nbx_get_launch_config": (_c_i, [_c_i, _c_i, _c_p, _c_p]),
+    "nbx_set_launch_config": (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i]),
+    "nbx_get_launch_config": (_c_i, [_c_i, _c_i, _c_p, _c_p, _c_p]),
+    "nbx_batch_create": (_c_i, [_c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_i64, _c_i, _c_p]),
+    "nbx_batch_particles": (_c_i64, [_c_p]),
+    "nbx_batch_run": (_c_i, [_c_p, _c_p]),
+    "nbx_batch_capture": (_c_i, [_c_p, _c_p]),
+    "nbx_batch_run_graph": (_c_i, [_c_p, _c_p]),
+    "nbx_batch_destroy": (None, [_c_p]),
+    "nbx_mlp_gemm": (_c_i, [_c_i, _c_p, _c_p, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "nbx_mlp_features": (_c_i, [_c_p, _c_i64, _c_p, _c_p, _c_p, _c_i, _c_p, _c_p]),
+    "nbx_mlp_out_forward": (_c_i, [_c_p, _c_i64, _c_i, _c_p, _c_p, _c_p, _c_p]),
+    "nbx_mlp_loss": (_c_i, [_c_p, _c_p, _c_i64, _c_p, _c_p, _c_p]),
+    "nbx_mlp_out_backward": (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_p, _c_p, _c_p, _c_p,
+                                    _c_p]),
+    "nbx_mlp_colsum": (_c_i, [_c_p, _c_i64, _c_i, _c_p, _c_p]),
+    "nbx_mlp_adam": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, ctypes.c_float, ctypes.c_float,
+                            ctypes.c_float, ctypes.c_float, _c_i, ctypes.c_float, _c_p]),
+    "nbx_mlp_to_bf16": (_c_i, [_c_p, _c_i64, _c_p, _c_p]),
     "nbx_ftle_grid_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_d, _c_p, _c_p, _c_p]),
     "nbx_C_eig_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_p, _c_p, _c_p, _c_p]),
     "nbx_ftle_eig_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_d, _c_p, _c_p, _c_p, _c_p]),
@@ -52,6 +68,13 @@ SIGNATURES = {
                           _c_p, _c_p, _c_p, _c_p]),
     "nbx_label8": (_c_i, [_c_p, _c_i, _c_i, _c_p, _c_p, _c_p]),
 }
+
+class Request(ctypes.Structure):
+    """struct nbx_request (include/numbacs_amd.h)."""
+    _fields_ = [("t0", _c_d), ("T", _c_d), ("x", _c_p), ("nx", _c_i), ("y", _c_p), ("ny", _c_i),
+                ("mask", _c_p), ("flowmap", _c_p), ("status", _c_p), ("nsteps", _c_p),
+                ("ftle", _c_p), ("eigvec_max", _c_p), ("dx", _c_d), ("dy", _c_d)]
+
 
 _lib = None
 

@@ -1,0 +1,81 @@
+"""Launch-shape autotuner for the flow-map kernels.
+
+For one (flow, precision, workload) it times every candidate launch shape --
+lanes per particle (spline flows: 1, 2, 4), threads per block (64, 128, 256)
+and persistent work-queue occupancy (0 = one lane group per particle, else
+blocks per compute unit) -- on the real workload, keeps the fastest, installs
+it with nbx_set_launch_config and caches it in a JSON file keyed by device
+name, flow kind, precision and workload size.
+"""
+import ctypes
+import json
+import os
+import time
+
+from . import _lib
+
+CACHE = os.path.join(os.path.expanduser("~"), ".cache", "numbacs_amd", "autotune.json")
+
+
+def candidates(kind, persistent=True):
+    lanes = (1, 2, 4) if kind == _lib.NBX_FLOW_SPLINE3 else (1,)
+    pers = (0, 2, 4, 8) if persistent else (0,)
+    return [(g, b, p) for g in lanes for b in (64, 128, 256) for p in pers]
+
+
+def get(kind, precision):
+    g, b, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _lib.check(_lib.load().nbx_get_launch_config(kind, precision, ctypes.byref(g), ctypes.byref(b),
+                                                 ctypes.byref(p)))
+    return g.value, b.value, p.value
+
+
+def set_config(kind, precision, cfg):
+    _lib.check(_lib.load().nbx_set_launch_config(kind, precision, *map(int, cfg)))
+
+
+def _key(kind, precision, size):
+    t = _lib.torch()
+    return f"{t.cuda.get_device_name()}|{kind}|{precision}|{size}"
+
+
+def _load_cache():
+    try:
+        with open(CACHE) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def tune(kind, precision, run, size, reps=3, persistent=True, use_cache=True):
+    """run() enqueues one flow map with the installed launch shape.  Returns
+    (best_cfg, {cfg: ms})."""
+    t = _lib.torch()
+    key = _key(kind, precision, size)
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        best = tuple(cache[key]["cfg"])
+        set_config(kind, precision, best)
+        return best, cache[key]["times"]
+    old = get(kind, precision)
+    times = {}
+    for cfg in candidates(kind, persistent):
+        set_config(kind, precision, cfg)
+        run()
+        t.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        t.cuda.synchronize()
+        times[",".join(map(str, cfg))] = 1e3 * (time.perf_counter() - t0) / reps
+    best_s = min(times, key=times.get)
+    best = tuple(int(v) for v in best_s.split(","))
+    set_config(kind, precision, best if best else old)
+    cache[key] = {"cfg": list(best), "times": times}
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, "w") as fh:
+            json.dump(cache, fh, indent=1)
+    except OSError:
+        pass
+    return best, times

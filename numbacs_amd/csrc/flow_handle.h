@@ -1,0 +1,81 @@
+// The native flow handle behind the C ABI and the host helpers that turn it
+// into a device functor; shared by the single-grid and batched launches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "flows.h"
+
+struct nbx_flow {
+    int kind;
+    // spline flows
+    double2* coef = nullptr;  // (nt+2, nx+2, ny+2) interleaved (u, v), device
+    float2* coef32 = nullptr; // FP32 copy for the FP32-displacement path
+    int64_t ncoef = 0;
+    double grid9[9] = {0};
+    int spherical = 0;
+    int extrap = 0;
+    double r = 6371.0;
+};
+
+namespace nbx {
+
+// Launch shape per (flow kind, precision), set by the autotuner through
+// nbx_set_launch_config:
+//   lanes       lanes per particle for spline flows (1, 2 or 4);
+//   block       threads per block (64, 128 or 256);
+//   persistent  0: one group per particle, one launch covering all of them;
+//               n > 0: a persistent grid of n blocks per CU pulling
+//               particles from a device work queue.
+struct LaunchCfg {
+    int lanes;
+    int block;
+    int persistent;
+};
+
+LaunchCfg launch_cfg(int kind, int precision);
+
+template <class R>
+Spline3<R, 1> make_spline(const nbx_flow* fl, double p0) {
+    Spline3<R, 1> s;
+    if constexpr (sizeof(R) == 8) s.C = fl->coef;
+    else s.C = fl->coef32;
+    for (int d = 0; d < 3; ++d) {
+        s.lo[d] = fl->grid9[3 * d];
+        s.hi[d] = fl->grid9[3 * d + 1];
+        const int n = static_cast<int>(fl->grid9[3 * d + 2]);
+        s.dinv[d] = (n - 1) / (s.hi[d] - s.lo[d]);
+        s.nm2[d] = n - 2;
+    }
+    const int nx = static_cast<int>(fl->grid9[5]), ny = static_cast<int>(fl->grid9[8]);
+    s.sx = ny + 2;
+    s.st = (nx + 2) * (ny + 2);
+    s.spherical = fl->spherical;
+    s.extrap = fl->extrap;
+    s.r = fl->r;
+    s.p0 = p0;
+    return s;
+}
+
+template <int G, class R>
+Spline3<R, G> regroup(const Spline3<R, 1>& a) {
+    Spline3<R, G> s;
+    s.C = a.C;
+    for (int d = 0; d < 3; ++d) {
+        s.lo[d] = a.lo[d];
+        s.hi[d] = a.hi[d];
+        s.dinv[d] = a.dinv[d];
+        s.nm2[d] = a.nm2[d];
+    }
+    s.sx = a.sx;
+    s.st = a.st;
+    s.spherical = a.spherical;
+    s.extrap = a.extrap;
+    s.r = a.r;
+    s.p0 = a.p0;
+    return s;
+}
+
+}  // namespace nbx

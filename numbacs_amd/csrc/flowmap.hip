@@ -9,25 +9,17 @@
 
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <string>
 #include <vector>
 
 #include "common.h"
 #include "dop853.h"
+#include "flow_handle.h"
 #include "flows.h"
-
-struct nbx_flow {
-    int kind;
-    // spline flows
-    double2* coef = nullptr;  // (nt+2, nx+2, ny+2) interleaved (u, v), device
-    float2* coef32 = nullptr; // FP32 copy for the FP32-displacement path
-    int64_t ncoef = 0;
-    double grid9[9] = {0};
-    int spherical = 0;
-    int extrap = 0;
-    double r = 6371.0;
-};
+#include "queue.h"
 
 namespace nbx {
 
@@ -241,61 +233,14 @@ __global__ void k_flow_eval(Flow f, const double* __restrict__ t, const double2*
     out[k] = make_double2(static_cast<double>(a), static_cast<double>(b));
 }
 
-// Launch shape per (flow kind, precision): lanes per particle for spline
-// flows (1, 2 or 4) and threads per block.  Set by the autotuner through
-// nbx_set_launch_config; the defaults are the measured best for the
-// gridded-velocity workload.
-struct LaunchCfg {
-    int lanes;
-    int block;
-};
-LaunchCfg g_cfg[3][2] = {{{1, 256}, {1, 256}}, {{1, 256}, {1, 256}}, {{1, 256}, {1, 256}}};
+LaunchCfg g_cfg[3][2] = {{{1, 256, 0}, {1, 256, 0}},
+                         {{1, 256, 0}, {1, 256, 0}},
+                         {{1, 256, 0}, {1, 256, 0}}};
 std::mutex g_cfg_mutex;
 
 LaunchCfg launch_cfg(int kind, int precision) {
     std::lock_guard<std::mutex> lock(g_cfg_mutex);
     return g_cfg[kind][precision];
-}
-
-template <class R>
-Spline3<R, 1> make_spline(const nbx_flow* fl, double p0) {
-    Spline3<R, 1> s;
-    if constexpr (sizeof(R) == 8) s.C = fl->coef;
-    else s.C = fl->coef32;
-    for (int d = 0; d < 3; ++d) {
-        s.lo[d] = fl->grid9[3 * d];
-        s.hi[d] = fl->grid9[3 * d + 1];
-        const int n = static_cast<int>(fl->grid9[3 * d + 2]);
-        s.dinv[d] = (n - 1) / (s.hi[d] - s.lo[d]);
-        s.nm2[d] = n - 2;
-    }
-    const int nx = static_cast<int>(fl->grid9[5]), ny = static_cast<int>(fl->grid9[8]);
-    s.sx = ny + 2;
-    s.st = (nx + 2) * (ny + 2);
-    s.spherical = fl->spherical;
-    s.extrap = fl->extrap;
-    s.r = fl->r;
-    s.p0 = p0;
-    return s;
-}
-
-template <int G, class R>
-Spline3<R, G> regroup(const Spline3<R, 1>& a) {
-    Spline3<R, G> s;
-    s.C = a.C;
-    for (int d = 0; d < 3; ++d) {
-        s.lo[d] = a.lo[d];
-        s.hi[d] = a.hi[d];
-        s.dinv[d] = a.dinv[d];
-        s.nm2[d] = a.nm2[d];
-    }
-    s.sx = a.sx;
-    s.st = a.st;
-    s.spherical = a.spherical;
-    s.extrap = a.extrap;
-    s.r = a.r;
-    s.p0 = a.p0;
-    return s;
 }
 
 struct SolveArgs {
@@ -308,8 +253,50 @@ struct SolveArgs {
     int32_t* nsteps;
 };
 
+int device_cu_count() {
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(mu);
+    if (static_cast<int>(cache.size()) <= dev) cache.resize(dev + 1, 0);
+    if (cache[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        cache[dev] = n;
+    }
+    return cache[dev];
+}
+
+// The single-grid request as a queue entry (grid sources only).
+inline ReqDev single_request(const GridSource& src, const SolveArgs& a) {
+    return ReqDev{a.s0, a.s1, src.xs, src.ys, static_cast<int>(a.P / src.ny), src.ny, a.mask,
+                  a.out, a.status, a.nsteps};
+}
+
 template <class Flow, class R, int G, class Src>
-int launch_solve(const Flow& f, const Src& src, const SolveArgs& a, int block, hipStream_t s) {
+int launch_solve(const Flow& f, const Src& src, const SolveArgs& a, const LaunchCfg& cfg,
+                 hipStream_t s) {
+    const int block = cfg.block;
+    if constexpr (std::is_same<Src, GridSource>::value) {
+        if (cfg.persistent > 0) {
+            const int cus = device_cu_count();
+            NBX_REQUIRE(cus > 0, "cannot query the compute-unit count");
+            const int64_t want = (a.P * G + block - 1) / block;
+            const int64_t grid = std::min<int64_t>(want, static_cast<int64_t>(cus) * cfg.persistent);
+            unsigned long long* counter = nullptr;
+            NBX_HIP(hipMallocAsync(reinterpret_cast<void**>(&counter), sizeof(*counter), s));
+            k_queue_reset<<<1, 1, 0, s>>>(counter);
+            NBX_LAUNCHED("queue_reset");
+            k_flowmap_queue<Flow, R, G><<<static_cast<unsigned>(grid), block, 0, s>>>(
+                f, single_request(src, a), nullptr, nullptr, 1, a.P, counter, a.rtol, a.atol,
+                a.nmax);
+            NBX_LAUNCHED("flowmap_queue");
+            NBX_HIP(hipFreeAsync(counter, s));
+            return 0;
+        }
+    }
     const int64_t threads = a.P * G;
     k_flowmap<Flow, R, G, Src><<<blocks_for(threads, block), block, 0, s>>>(
         f, src, a.P, a.s0, a.s1, a.rtol, a.atol, a.mask, a.nmax, a.out, a.status, a.nsteps);
@@ -321,24 +308,22 @@ template <class R, class Src>
 int dispatch_solve(const nbx_flow* fl, const double* params, int np, const Src& src,
                    const SolveArgs& a, int precision, hipStream_t s) {
     const LaunchCfg cfg = launch_cfg(fl->kind, precision);
-    int block = cfg.block;
-    if (block != 64 && block != 128 && block != 256) block = 256;
     if (fl->kind == NBX_FLOW_SPLINE3) {
         const Spline3<R, 1> sp = make_spline<R>(fl, params[0]);
-        if (cfg.lanes == 4) return launch_solve<Spline3<R, 4>, R, 4>(regroup<4>(sp), src, a, block, s);
-        if (cfg.lanes == 2) return launch_solve<Spline3<R, 2>, R, 2>(regroup<2>(sp), src, a, block, s);
-        return launch_solve<Spline3<R, 1>, R, 1>(sp, src, a, block, s);
+        if (cfg.lanes == 4) return launch_solve<Spline3<R, 4>, R, 4>(regroup<4>(sp), src, a, cfg, s);
+        if (cfg.lanes == 2) return launch_solve<Spline3<R, 2>, R, 2>(regroup<2>(sp), src, a, cfg, s);
+        return launch_solve<Spline3<R, 1>, R, 1>(sp, src, a, cfg, s);
     }
     if (fl->kind == NBX_FLOW_DOUBLE_GYRE) {
         NBX_REQUIRE(np >= 6, "double_gyre needs 6 parameters [dir, A, eps, alpha, omega, psi]");
         const DoubleGyre<R> dg{params[0], params[1], params[2], params[3], params[4], params[5]};
-        return launch_solve<DoubleGyre<R>, R, 1>(dg, src, a, block, s);
+        return launch_solve<DoubleGyre<R>, R, 1>(dg, src, a, cfg, s);
     }
     if (fl->kind == NBX_FLOW_BICKLEY_JET) {
         NBX_REQUIRE(np >= 12, "bickley_jet needs 12 parameters");
         BickleyJet<R> bj;
         for (int k = 0; k < 12; ++k) bj.p[k] = params[k];
-        return launch_solve<BickleyJet<R>, R, 1>(bj, src, a, block, s);
+        return launch_solve<BickleyJet<R>, R, 1>(bj, src, a, cfg, s);
     }
     set_error("unknown flow kind");
     return -2;
@@ -503,7 +488,8 @@ int nbx_flowmap_pts(const nbx_flow* fl, double t0, double T, const double* pts, 
                  as_stream(stream));
 }
 
-int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int block_threads) {
+int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int block_threads,
+                          int persistent_blocks_per_cu) {
     NBX_REQUIRE(kind >= 0 && kind <= 2, "unknown flow kind");
     NBX_REQUIRE(precision == NBX_FP64 || precision == NBX_FP32_DISP, "unknown precision");
     NBX_REQUIRE(lanes_per_particle == 1 || lanes_per_particle == 2 || lanes_per_particle == 4,
@@ -512,18 +498,24 @@ int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int b
                 "block_threads must be 64, 128 or 256");
     NBX_REQUIRE(kind == NBX_FLOW_SPLINE3 || lanes_per_particle == 1,
                 "analytic flows run one lane per particle");
+    NBX_REQUIRE(persistent_blocks_per_cu >= 0 && persistent_blocks_per_cu <= 64,
+                "persistent_blocks_per_cu must be in [0, 64]");
     std::lock_guard<std::mutex> lock(g_cfg_mutex);
-    g_cfg[kind][precision] = LaunchCfg{lanes_per_particle, block_threads};
+    g_cfg[kind][precision] = LaunchCfg{lanes_per_particle, block_threads, persistent_blocks_per_cu};
     return 0;
 }
 
-int nbx_get_launch_config(int kind, int precision, int* lanes_per_particle, int* block_threads) {
+int nbx_get_launch_config(int kind, int precision, int* lanes_per_particle, int* block_threads,
+                          int* persistent_blocks_per_cu) {
     NBX_REQUIRE(kind >= 0 && kind <= 2, "unknown flow kind");
     NBX_REQUIRE(precision == NBX_FP64 || precision == NBX_FP32_DISP, "unknown precision");
-    NBX_REQUIRE(lanes_per_particle != nullptr && block_threads != nullptr, "NULL output");
+    NBX_REQUIRE(lanes_per_particle != nullptr && block_threads != nullptr &&
+                    persistent_blocks_per_cu != nullptr,
+                "NULL output");
     const LaunchCfg c = launch_cfg(kind, precision);
     *lanes_per_particle = c.lanes;
     *block_threads = c.block;
+    *persistent_blocks_per_cu = c.persistent;
     return 0;
 }
 

@@ -96,3 +96,20 @@ def field_sha256(U, V):
     h.update(np.ascontiguousarray(U, dtype=np.float64).tobytes())
     h.update(np.ascontiguousarray(V, dtype=np.float64).tobytes())
     return h.hexdigest()
+
+
+def time_series_requests(nreq=48, seed=FIELD_SEED + 1, t0_first=72.0,
+                         windows=(-24.0, -48.0, -72.0), nx_range=(201, 1351),
+                         ny_range=(101, 501)):
+    """FTLE time-series requests over the gridded field extended to 480
+    hourly frames: hourly t0 from 72 h, window lengths cycling through
+    -24/-48/-72 h, particle grids of seeded uneven sizes between 201x101 and
+    1351x501 over the velocity box."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nreq):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        out.append(dict(t0=t0_first + k, T=windows[k % len(windows)],
+                        x=np.linspace(-100.0, 35.0, nx), y=np.linspace(-5.0, 45.0, ny)))
+    return out

@@ -1,0 +1,173 @@
+"""Batched time series (persistent work-queue launch, batched FTLE, HIP
+graph replay), the persistent single-grid launch and the FP32-displacement
+form, all through the C ABI.
+
+The queue kernel runs the same per-particle solver as the one-group-per-
+particle kernel, so with the same lanes-per-particle setting its flow maps
+must be bit-identical to flowmap_grid_2D's; FTLE fields from the batch must
+be bit-identical to ftle_grid_2D / the fused eigen stencil on those maps.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from numbacs_amd import _lib
+
+    _lib.torch()
+    return _lib
+
+
+@pytest.fixture
+def restore_config(lib):
+    from numbacs_amd import autotune
+
+    saved = {(k, p): autotune.get(k, p) for k in (0, 1, 2) for p in (0, 1)}
+    yield autotune
+    for (k, p), cfg in saved.items():
+        autotune.set_config(k, p, cfg)
+
+
+def _dg_requests():
+    return [dict(t0=0.0, T=-10.0, x=np.linspace(0, 2, 101), y=np.linspace(0, 1, 51)),
+            dict(t0=2.5, T=-5.0, x=np.linspace(0, 2, 37), y=np.linspace(0, 1, 73)),
+            dict(t0=1.0, T=8.0, x=np.linspace(0.2, 1.8, 64), y=np.linspace(0.1, 0.9, 1)),
+            dict(t0=0.0, T=-3.0, x=np.linspace(0, 2, 5), y=np.linspace(0, 1, 3))]
+
+
+@pytest.mark.parametrize("block,persist", [(64, 2), (256, 8)])
+def test_batch_matches_single_grid_double_gyre(lib, restore_config, block, persist):
+    from numbacs_amd.batch import FTLEBatch
+    from numbacs_amd.diagnostics import ftle_grid_2D
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    restore_config.set_config(0, 0, (1, block, persist))
+    f, p, _ = get_predefined_flow("double_gyre", int_direction=-1.0)
+    reqs = _dg_requests()
+    b = FTLEBatch(f, reqs, p).run()
+    res = b.results()
+    assert b.particles == sum(len(q["x"]) * len(q["y"]) for q in reqs)
+    for q, r in zip(reqs, res):
+        fm, st = flowmap_grid_2D(f, q["t0"], q["T"], q["x"], q["y"], p, return_status=True)
+        assert np.array_equal(r["flowmap"], fm)
+        assert np.array_equal(r["status"], st)
+        dx = q["x"][1] - q["x"][0]
+        dy = q["y"][1] - q["y"][0] if len(q["y"]) > 1 else 1.0
+        assert np.array_equal(r["ftle"], ftle_grid_2D(fm, q["T"], dx, dy))
+    b.close()
+
+
+def test_batch_matches_oracle_double_gyre(lib):
+    from numbacs_amd.batch import FTLEBatch
+    from numbacs_amd.flows import get_predefined_flow
+
+    f, p, _ = get_predefined_flow("double_gyre", int_direction=-1.0)
+    reqs = _dg_requests()[:2]
+    res = FTLEBatch(f, reqs, p).run().results()
+    for q, r in zip(reqs, res):
+        ref, _, _ = oracle.flowmap_grid_2d("double_gyre", p, q["t0"], q["T"], q["x"], q["y"])
+        assert np.abs(r["flowmap"] - ref).max() / np.abs(ref).max() < 1e-6
+
+
+@pytest.fixture(scope="module")
+def small_field(lib):
+    from numbacs_amd.flows import get_flow_2D, get_interp_arrays_2D
+    from numbacs_amd.workloads import synthetic_velocity
+
+    t, lon, lat, U, V = synthetic_velocity(nt=40)
+    grid, Cu, Cv = get_interp_arrays_2D(t, lon, lat, U, V)
+    return grid, Cu, Cv, get_flow_2D(grid, Cu, Cv, spherical=1)
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_batch_matches_single_grid_spline(lib, restore_config, small_field, lanes):
+    from numbacs_amd.batch import FTLEBatch
+    from numbacs_amd.diagnostics import C_eig_2D, ftle_from_eig
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    restore_config.set_config(2, 0, (lanes, 64, 4))
+    grid, Cu, Cv, f = small_field
+    reqs = [dict(t0=30.0 + k, T=(-6.0, -12.0, -24.0)[k % 3], x=np.linspace(-100, 35, 60 + 7 * k),
+                 y=np.linspace(-5, 45, 25 + 3 * k)) for k in range(5)]
+    reqs[1]["mask"] = np.zeros((len(reqs[1]["x"]), len(reqs[1]["y"])), bool)
+    reqs[1]["mask"][::3, ::2] = True
+    b = FTLEBatch(f, reqs, [-1.0], eigvec=True).run()
+    for q, r in zip(reqs, b.results()):
+        fm = flowmap_grid_2D(f, q["t0"], q["T"], q["x"], q["y"], [-1.0], mask=q.get("mask"))
+        assert np.array_equal(r["flowmap"], fm)
+        dx, dy = q["x"][1] - q["x"][0], q["y"][1] - q["y"][0]
+        ev, V = C_eig_2D(fm, dx, dy, mask=q.get("mask"))
+        assert np.array_equal(r["eigvec_max"], V[:, :, :, 1])
+        assert np.allclose(r["ftle"], ftle_from_eig(ev[:, :, 1], q["T"]), rtol=1e-14, atol=0)
+    b.close()
+
+
+def test_graph_replays_match_eager(lib, small_field):
+    """The captured graph replayed several times, with eager work and
+    synchronisation between replays, reproduces the eager batch exactly."""
+    import torch
+
+    from numbacs_amd.batch import FTLEBatch
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    grid, Cu, Cv, f = small_field
+    reqs = [dict(t0=20.0 + 2 * k, T=-12.0, x=np.linspace(-100, 35, 50), y=np.linspace(-5, 45, 31))
+            for k in range(4)]
+    b = FTLEBatch(f, reqs, [-1.0]).run()
+    eager = b.results()
+    b.capture()
+    for rep in range(3):
+        for d in b.requests:
+            d["flowmap"].fill_(0.0)
+            d["ftle"].fill_(-1.0)
+        torch.cuda.synchronize()
+        b.run_graph()
+        torch.cuda.synchronize()
+        got = b.results()
+        for e, g in zip(eager, got):
+            assert np.array_equal(e["flowmap"], g["flowmap"]), rep
+            assert np.array_equal(e["ftle"], g["ftle"]), rep
+        # unrelated eager work between replays
+        flowmap_grid_2D(f, 25.0, -6.0, np.linspace(-90, 20, 17), np.linspace(0, 40, 9), [-1.0])
+    b.close()
+
+
+@pytest.mark.parametrize("lanes,block,persist", [(2, 64, 4), (4, 128, 2), (1, 256, 8)])
+def test_persistent_single_grid_matches(lib, restore_config, small_field, lanes, block, persist):
+    from numbacs_amd.integration import flowmap_grid_2D
+
+    grid, Cu, Cv, f = small_field
+    x, y = np.linspace(-100, 35, 140), np.linspace(-5, 45, 61)
+    restore_config.set_config(2, 0, (lanes, block, 0))
+    a, sa = flowmap_grid_2D(f, 35.0, -30.0, x, y, [-1.0], return_status=True)
+    restore_config.set_config(2, 0, (lanes, block, persist))
+    b, sb = flowmap_grid_2D(f, 35.0, -30.0, x, y, [-1.0], return_status=True)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+
+
+def test_fp32_displacement_error_double_gyre(lib):
+    """FP32-displacement form against the FP64 oracle on the double gyre
+    (401x201, T=-10): flow-map and FTLE errors stay within the stated
+    bounds (documented with the measured values)."""
+    from numbacs_amd.diagnostics import ftle_grid_2d_device
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2d_device
+
+    f, p, _ = get_predefined_flow("double_gyre", int_direction=-1.0)
+    x, y = np.linspace(0, 2, 401), np.linspace(0, 1, 201)
+    fm = flowmap_grid_2d_device(f, 0.0, -10.0, lib.to_dev(x), lib.to_dev(y), p,
+                                precision=lib.NBX_FP32_DISP)
+    ft = lib.to_host(ftle_grid_2d_device(fm, -10.0, 0.005, 0.005))
+    ref, _, _ = oracle.flowmap_grid_2d("double_gyre", p, 0.0, -10.0, x, y)
+    ftr = oracle.ftle_grid_2d(ref, -10.0, 0.005, 0.005)
+    fm = lib.to_host(fm)
+    assert np.abs(fm - ref).max() < 1e-3
+    err = np.abs(ft - ftr) / np.abs(ftr).max()
+    assert np.sqrt(np.mean(err ** 2)) < 1e-4
+    assert np.mean(err <= 1e-3) > 0.995
