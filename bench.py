@@ -1,17 +1,23 @@
-"""FTLE benchmark on one MI355X: gridded-velocity FTLE + ridges, FP64.
+"""FTLE benchmark on one MI355X.
 
-Workload (one step): a seeded 64-mode synthetic velocity field on a
+Default workload (the one the headline metric is quoted on): gridded-velocity
+FTLE + ridges in FP64.  A seeded 64-mode synthetic velocity field on a
 0.5-degree lon/lat grid of 271x101 points with 96 hourly frames, tricubic
 spline (spherical=1, km/h), particles on a 0.2-degree grid (676x251),
 t0 = 72 h, T = -72 h, DOP853 rtol=1e-6 atol=1e-8 in FP64, then the fused
 Cauchy-Green eigen/FTLE stencil and FTLE ridge extraction at the 90th
-percentile.  Inputs (spline coefficients, particle axes) are resident in HBM
-before timing starts.
+percentile.  One step = flow map + FTLE + ridges with inputs resident in HBM.
 
-Prints one JSON line: grid points/s, the dominant kernel's roofline figures
-(its launch duration measured with HIP events on the stream it runs on),
-and the CPU oracle timed on this machine's host cores as a reported
-baseline, plus the error of the GPU result against that FP64 oracle.
+The flow-map launch shape is chosen first by the autotuner (timed on this
+workload, outside the timed region).  The JSON line carries the dominant
+kernel's roofline figures (its duration measured with HIP events on the
+stream it runs on) and the CPU oracle timed on this machine's host cores,
+with the error of the GPU result against that FP64 oracle.
+
+Other workloads (--workload): double_gyre (401x201, T=-10), timeseries
+(48 uneven requests over a 480-frame field in one batched launch, FP64 and
+FP32-displacement), precision (double gyre 2001x1001 and Bickley 2001x801 in
+FP32-displacement vs FP64), surrogate (MLP surrogate + DOP853 fallback).
 """
 import argparse
 import json
@@ -28,13 +34,16 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
 # algorithmic figures per unit of work
-SPLINE_BYTES_PER_EVAL = 2 * 64 * 8         # 64 (u, v) coefficient pairs, FP64
-EVALS_PER_STEP = 12                        # DOP853 stages per attempted step
+SPLINE_BYTES_PER_EVAL = 2 * 64 * 8          # 64 (u, v) coefficient pairs, FP64
+EVALS_PER_STEP = 12                         # DOP853 stages per attempted step
 BYTES_PER_PARTICLE_STEP = SPLINE_BYTES_PER_EVAL * EVALS_PER_STEP  # 12,288 B
-FLOPS_RK_PER_STEP = 350                    # stage sums, solution, error norm, control
-FLOPS_SPLINE_RHS = 400                     # weights + 2 x 64 FMA + spherical scaling
+FLOPS_RK_PER_STEP = 350                     # stage sums, solution, error norm, control
+FLOPS_SPLINE_RHS = 400                      # weights + 2 x 64 FMA + spherical scaling
 FLOPS_PER_PARTICLE_STEP = FLOPS_RK_PER_STEP + EVALS_PER_STEP * FLOPS_SPLINE_RHS
+FLOPS_DG_RHS = 35                           # 30 flops + 5 transcendentals
+MLP_FLOPS_PER_SAMPLE = 2 * (128 * 256 + 3 * 256 * 256 + 256 * 2)
 
 
 def parse():
@@ -44,40 +53,51 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--workload", default="gridded",
+                    choices=["gridded", "double_gyre", "timeseries", "precision", "surrogate"])
+    ap.add_argument("--surrogate-samples", type=int, default=4_000_000)
+    ap.add_argument("--surrogate-t0s", type=int, default=2)
     return ap.parse_args()
 
 
-def build_workload(lib):
-    from numbacs_amd.flows import get_flow_2D, prefilter3_device
-    from numbacs_amd.workloads import field_sha256, particle_axes, synthetic_velocity
+def timed(fn, steps, warmup):
+    import torch
 
-    t, lon, lat, U, V = synthetic_velocity(nt=96)
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def gridded_field(nt=96):
+    from numbacs_amd.flows import get_flow_2D, prefilter3_device
+    from numbacs_amd.workloads import field_sha256, synthetic_velocity
+
+    t, lon, lat, U, V = synthetic_velocity(nt=nt)
     grid = ((t[0], t[-1], len(t)), (lon[0], lon[-1], len(lon)), (lat[0], lat[-1], len(lat)))
     Cu = prefilter3_device(U)
     Cv = prefilter3_device(V)
     flow = get_flow_2D(grid, Cu, Cv, spherical=1)
-    xp, yp = particle_axes(0.2)
-    return dict(t=t, lon=lon, lat=lat, U=U, V=V, grid=grid, Cu=Cu, Cv=Cv, flow=flow, xp=xp, yp=yp,
-                sha=field_sha256(U, V))
+    return dict(t=t, lon=lon, lat=lat, grid=grid, Cu=Cu, Cv=Cv, flow=flow, sha=field_sha256(U, V))
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is design-only)"}))
-        sys.exit(2)
-
+# ---------------------------------------------------------------------------
+def bench_gridded(args):
     import torch
 
-    from numbacs_amd import _lib
+    from numbacs_amd import _lib, autotune
     from numbacs_amd.diagnostics import ftle_eig_2d_device
     from numbacs_amd.extraction.ridges import ftle_ridges_device
     from numbacs_amd.integration import flowmap_grid_2d_device
+    from numbacs_amd.workloads import particle_axes
 
-    torch.cuda.set_device(0)
-    W = build_workload(_lib)
-    xp, yp = W["xp"], W["yp"]
+    W = gridded_field(96)
+    xp, yp = particle_axes(0.2)
     nx, ny = len(xp), len(yp)
     xd, yd = _lib.to_dev(xp), _lib.to_dev(yp)
     params = np.array([-1.0])
@@ -89,26 +109,31 @@ def main():
     ftle = _lib.empty((nx, ny))
     evec = _lib.empty((nx, ny, 2))
 
-    ev_fm = []
-
-    def step(record=False):
-        if record:
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
+    def flowmap():
         flowmap_grid_2d_device(W["flow"], t0, T, xd, yd, params, 1e-6, 1e-8, out=fm,
                                status=status, nsteps=nsteps)
-        if record:
-            e1.record()
-            ev_fm.append((e0, e1))
+
+    tuned, tune_ms = (autotune.get(2, 0), None) if args.no_tune else autotune.tune(
+        _lib.NBX_FLOW_SPLINE3, _lib.NBX_FP64, flowmap, nx * ny, use_cache=False)
+
+    ev_fm = []
+
+    def step():
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        flowmap()
+        e1.record()
+        ev_fm.append((e0, e1))
         ftle_eig_2d_device(fm, T, dx, dy, ftle=ftle, evec=evec)
         return ftle_ridges_device(ftle, evec, xd, yd, nx, ny, dx, dy, 0.0, 90, 3)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    ev_fm.clear()
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        res = step(record=True)
+        res = step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     ms_step = 1e3 * wall / args.steps
@@ -117,18 +142,16 @@ def main():
     P = nx * ny
     steps_total = int(nsteps.sum().item())
     st = _lib.to_host(status)
-    pts_per_s = P / (wall / args.steps)
     bytes_launch = BYTES_PER_PARTICLE_STEP * steps_total
     flops_launch = FLOPS_PER_PARTICLE_STEP * steps_total
     achieved_gbs = bytes_launch / (fm_ms * 1e-3) / 1e9
     achieved_tflops = flops_launch / (fm_ms * 1e-3) / 1e12
-
     fm_h = _lib.to_host(fm)
     ftle_h = _lib.to_host(ftle)
     lon, lat = W["lon"], W["lat"]
     outside = ((fm_h[..., 0] < lon[0]) | (fm_h[..., 0] > lon[-1]) | (fm_h[..., 1] < lat[0]) |
                (fm_h[..., 1] > lat[-1]))
-    pts_out, offsets, nridges, nridge_pts = res
+    _, _, nridges, nridge_pts = res
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "flowmap_spline_gridded_676x251_hbm.json")
@@ -138,7 +161,7 @@ def main():
 
     rec = {
         "metric": "FTLE grid points/s (gridded-velocity FTLE + ridges, plain FP64 DOP853)",
-        "value": pts_per_s,
+        "value": P / (wall / args.steps),
         "unit": "grid points/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -156,10 +179,12 @@ def main():
             "particles": P,
             "field_sha256": W["sha"],
             "parallelism": "single GPU",
+            "launch": {"lanes_per_particle": tuned[0], "block": tuned[1],
+                       "persistent_blocks_per_cu": tuned[2]},
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_flowmap_grid<Spline3<double>>",
+            "kernel": "DOP853 flow map over the tricubic spline (k_flowmap / k_flowmap_queue)",
             "achieved": achieved_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -176,15 +201,15 @@ def main():
         "final_outside_velocity_grid": int(outside.sum()),
         "ridges": int(nridges),
         "ridge_points": int(nridge_pts),
+        "autotune_ms": tune_ms,
     }
-
     if not args.no_cpu:
-        rec["cpu_baseline"], rec["error_vs_fp64_oracle"] = cpu_baseline(W, fm_h, ftle_h, t0, T,
-                                                                         args.cpu_reps)
-    print(json.dumps(rec))
+        rec["cpu_baseline"], rec["error_vs_fp64_oracle"] = cpu_baseline(W, xp, yp, fm_h, ftle_h,
+                                                                         t0, T, args.cpu_reps)
+    return rec
 
 
-def cpu_baseline(W, fm_gpu, ftle_gpu, t0, T, reps):
+def cpu_baseline(W, xp, yp, fm_gpu, ftle_gpu, t0, T, reps):
     """The CPU oracle (C restatement of the reference algorithm, OpenMP over
     grid rows as the reference's prange) on the full workload, on all host
     cores this process may use; median of reps runs after one warm-up."""
@@ -192,10 +217,9 @@ def cpu_baseline(W, fm_gpu, ftle_gpu, t0, T, reps):
 
     cores = len(os.sched_getaffinity(0))
     S = oracle.Spline(W["grid"], _host(W["Cu"]), _host(W["Cv"]), spherical=1)
-    xp, yp = W["xp"], W["yp"]
     dx, dy = xp[1] - xp[0], yp[1] - yp[0]
     times = []
-    for r in range(reps + 1):
+    for _ in range(reps + 1):
         t1 = time.perf_counter()
         fm, st, _ = oracle.flowmap_grid_2d("spline", np.array([-1.0]), t0, T, xp, yp, spline=S,
                                            nthreads=cores)
@@ -205,22 +229,245 @@ def cpu_baseline(W, fm_gpu, ftle_gpu, t0, T, reps):
         times.append(time.perf_counter() - t1)
     med = float(np.median(times[1:]))
     P = len(xp) * len(yp)
-    scale = np.abs(fm).max()
-    ferr = np.abs(fm_gpu - fm).max(-1) / scale
-    fscale = np.abs(ftle).max()
-    ftle_err = np.abs(ftle_gpu - ftle) / fscale
+    ferr = np.abs(fm_gpu - fm).max(-1) / np.abs(fm).max()
+    ftle_err = np.abs(ftle_gpu - ftle) / np.abs(ftle).max()
     cpu = {"value": P / med, "unit": "grid points/s", "cores": cores, "kind": "port",
            "sample": f"full workload ({P} particles, flow map + FTLE + ridges), median of {reps}",
            "seconds": med}
     err = {"flowmap_max_rel": float(ferr.max()),
            "flowmap_frac_within_1e-6": float(np.mean(ferr <= 1e-6)),
            "ftle_max_rel": float(ftle_err.max()),
+           "ftle_rms_rel": float(np.sqrt(np.mean(ftle_err ** 2))),
            "ftle_frac_within_1e-6": float(np.mean(ftle_err <= 1e-6))}
     return cpu, err
 
 
 def _host(t):
     return t.detach().cpu().numpy()
+
+
+# ---------------------------------------------------------------------------
+def bench_double_gyre(args):
+    from numbacs_amd import _lib, autotune
+    from numbacs_amd.diagnostics import ftle_grid_2d_device
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2d_device
+
+    f, p, _ = get_predefined_flow("double_gyre", int_direction=-1.0)
+    x, y = np.linspace(0, 2, 401), np.linspace(0, 1, 201)
+    xd, yd = _lib.to_dev(x), _lib.to_dev(y)
+    fm = _lib.empty((401, 201, 2))
+    ns = _lib.empty((401, 201), "i32")
+
+    def flow():
+        flowmap_grid_2d_device(f, 0.0, -10.0, xd, yd, p, out=fm, nsteps=ns)
+
+    tuned, _ = autotune.tune(_lib.NBX_FLOW_DOUBLE_GYRE, _lib.NBX_FP64, flow, 401 * 201,
+                             use_cache=False)
+
+    def step():
+        flow()
+        return ftle_grid_2d_device(fm, -10.0, 0.005, 0.005)
+
+    sec, _ = timed(step, args.steps, args.warmup)
+    steps_total = int(ns.sum().item())
+    return {"workload": "double gyre 401x201, t0=0, T=-10, DOP853 FP64 + FTLE",
+            "grid_points_per_s": 401 * 201 / sec, "ms_per_step": 1e3 * sec,
+            "particle_steps_per_s": steps_total / sec, "launch": tuned}
+
+
+# ---------------------------------------------------------------------------
+def bench_timeseries(args):
+    import torch
+
+    import oracle
+    from numbacs_amd import _lib, autotune
+    from numbacs_amd.batch import FTLEBatch
+    from numbacs_amd.workloads import time_series_requests
+
+    W = gridded_field(480)
+    reqs = time_series_requests()
+    out = {"workload": "FTLE time series: 48 requests (t0 = 72..119 h hourly, T cycling "
+                       "-24/-48/-72 h, seeded grids 201x101..1351x501) over a 271x101x480 "
+                       "field, one persistent launch + batched FTLE, HIP graph replay"}
+    P = sum(len(q["x"]) * len(q["y"]) for q in reqs)
+    out["particles"] = P
+    results = {}
+    for prec in ("fp64", "fp32"):
+        b = FTLEBatch(W["flow"], reqs, [-1.0], precision=prec)
+        pc = _lib.NBX_FP64 if prec == "fp64" else _lib.NBX_FP32_DISP
+        tuned, _ = autotune.tune(_lib.NBX_FLOW_SPLINE3, pc, b.run, P, reps=1, use_cache=False)
+        b.capture()
+        sec, _ = timed(b.run_graph, max(1, args.steps // 4), 1)
+        results[prec] = b.results()
+        out[prec] = {"grid_points_per_s": P / sec, "seconds": sec, "launch": tuned,
+                     "particle_steps": b.total_steps()}
+        b.close()
+    # FP32 against FP64 on every request, and FP64 against the CPU oracle on two
+    ferr, ftle_err = [], []
+    for r64, r32 in zip(results["fp64"], results["fp32"]):
+        ferr.append(np.abs(r32["flowmap"] - r64["flowmap"]).max())
+        ftle_err.append(np.abs(r32["ftle"] - r64["ftle"]).max() / np.abs(r64["ftle"]).max())
+    out["fp32_vs_fp64"] = {"flowmap_max_abs_deg": float(max(ferr)),
+                           "ftle_max_rel": float(max(ftle_err)),
+                           "ftle_median_request_max_rel": float(np.median(ftle_err))}
+    S = oracle.Spline(W["grid"], _host(W["Cu"]), _host(W["Cv"]), spherical=1)
+    oerr = []
+    for k in (0, 1):
+        q = reqs[k]
+        ref, _, _ = oracle.flowmap_grid_2d("spline", np.array([-1.0]), q["t0"], q["T"], q["x"],
+                                           q["y"], spline=S)
+        for prec in ("fp64", "fp32"):
+            oerr.append((prec, float(np.abs(results[prec][k]["flowmap"] - ref).max(-1).max()
+                                     / np.abs(ref).max())))
+    out["vs_fp64_oracle_first_two_requests"] = oerr
+    out["fp32_speedup"] = out["fp32"]["grid_points_per_s"] / out["fp64"]["grid_points_per_s"]
+    return out
+
+
+# ---------------------------------------------------------------------------
+def bench_precision(args):
+    import oracle
+    from numbacs_amd import _lib
+    from numbacs_amd.diagnostics import ftle_eig_2d_device
+    from numbacs_amd.extraction.ridges import ftle_ridge_pts_device
+    from numbacs_amd.flows import get_predefined_flow
+    from numbacs_amd.integration import flowmap_grid_2d_device
+    from numbacs_amd.workloads import bickley_grid, bickley_params, double_gyre_params
+
+    cases = [("double_gyre", 2001, 1001, -10.0), ("double_gyre", 2001, 1001, -20.0),
+             ("bickley_jet", 2001, 801, -40.0)]
+    out = {"workload": "precision study: FP32-displacement vs FP64 (GPU) and the FP64 oracle "
+                       "on sampled rows", "cases": []}
+    for name, nx, ny, T in cases:
+        if name == "double_gyre":
+            f = get_predefined_flow(name, return_default_params=False, return_domain=False)
+            p = double_gyre_params(-1.0)
+            x, y = np.linspace(0, 2, nx), np.linspace(0, 1, ny)
+        else:
+            f = get_predefined_flow(name, return_default_params=False, return_domain=False)
+            p = bickley_params(-1.0)
+            x, y = bickley_grid(nx, ny)
+        xd, yd = _lib.to_dev(x), _lib.to_dev(y)
+        dx, dy = x[1] - x[0], y[1] - y[0]
+        rec = {"flow": name, "grid": [nx, ny], "T": T}
+        res = {}
+        for prec, code in (("fp64", 0), ("fp32", 1)):
+            fm = _lib.empty((nx, ny, 2))
+
+            def run():
+                flowmap_grid_2d_device(f, 0.0, T, xd, yd, p, precision=code, out=fm)
+
+            sec, _ = timed(run, 3, 1)
+            ftle, ev = ftle_eig_2d_device(fm, T, dx, dy)
+            pts, npts = ftle_ridge_pts_device(ftle, ev, xd, yd, nx, ny, dx, dy, 0.0, 90)
+            res[prec] = (_lib.to_host(fm), _lib.to_host(ftle), _lib.to_host(pts[:npts]))
+            rec[prec + "_grid_points_per_s"] = nx * ny / sec
+        e = np.abs(res["fp32"][1] - res["fp64"][1]) / np.abs(res["fp64"][1]).max()
+        rec["fp32_ftle_max_rel"] = float(e.max())
+        rec["fp32_ftle_rms_rel"] = float(np.sqrt(np.mean(e ** 2)))
+        rec["fp32_ftle_frac_within_1e-3"] = float(np.mean(e <= 1e-3))
+        a, b = res["fp64"][2], res["fp32"][2]
+        rec["ridge_points_fp64"], rec["ridge_points_fp32"] = int(len(a)), int(len(b))
+        # oracle on every 50th row (flow map only)
+        rows = np.arange(0, nx, 50)
+        kind = name
+        ref, _, _ = oracle.flowmap_grid_2d(kind, p, 0.0, T, x[rows], y)
+        for prec in ("fp64", "fp32"):
+            d = np.abs(res[prec][0][rows] - ref).max(-1) / np.abs(ref).max()
+            rec[prec + "_vs_oracle_flowmap_max_rel"] = float(d.max())
+            rec[prec + "_vs_oracle_frac_within_1e-6"] = float(np.mean(d <= 1e-6))
+        out["cases"].append(rec)
+    return out
+
+
+# ---------------------------------------------------------------------------
+def bench_surrogate(args):
+    import torch
+
+    from numbacs_amd import _lib
+    from numbacs_amd import surrogate as S
+    from numbacs_amd.diagnostics import ftle_grid_2d_device
+    from numbacs_amd.integration import flowmap_grid_2d_device
+
+    W = gridded_field(96)
+    box = ((-100.0, 35.0), (-5.0, 45.0))
+    per = 100
+    nreq = max(1, args.surrogate_samples // (per * per))
+    t_gen = time.perf_counter()
+    reqs = S.sample_requests(nreq, per, box, (72.0, 95.0), (-72.0, -72.0), seed=5)
+    inp, tgt = S.integrate_samples(W["flow"], [-1.0], reqs)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+    net = S.MLPSurrogate([-32.5, 20.0, 83.5, -72.0], [67.5, 25.0, 11.5, 1.0], [15.0, 15.0],
+                         sigma=2.0, seed=7)
+    t_train = time.perf_counter()
+    losses = S.train(net, inp, tgt, epochs=3, batch=8192, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t_train
+    steps = 3 * (int(inp.shape[0]) // 8192)
+    train_tflops = 3 * MLP_FLOPS_PER_SAMPLE * 8192 * steps / t_train / 1e12
+
+    nx, ny = 2701, 1001
+    x, y = np.linspace(-100, 35, nx), np.linspace(-5, 45, ny)
+    dx = x[1] - x[0]
+    t0s = np.linspace(72.0, 95.75, 96)[:: max(1, 96 // args.surrogate_t0s)][: args.surrogate_t0s]
+    xd, yd = _lib.to_dev(x), _lib.to_dev(y)
+    plain_s, hyb_s, infos, errs, mlp_s = 0.0, 0.0, [], [], 0.0
+    for t0 in t0s:
+        fm_ref = _lib.empty((nx, ny, 2))
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        flowmap_grid_2d_device(W["flow"], t0, -72.0, xd, yd, [-1.0], out=fm_ref)
+        ft_ref = ftle_grid_2d_device(fm_ref, -72.0, dx, dx)
+        torch.cuda.synchronize()
+        plain_s += time.perf_counter() - a
+        a = time.perf_counter()
+        inp_g = S.grid_inputs(x, y, t0, -72.0)
+        net.predict_device(inp_g)
+        torch.cuda.synchronize()
+        mlp_s += time.perf_counter() - a
+        a = time.perf_counter()
+        fm, info = S.flowmap_with_fallback(net, W["flow"], [-1.0], x, y, t0, -72.0, 0.05 * dx)
+        ft = ftle_grid_2d_device(fm, -72.0, dx, dx)
+        torch.cuda.synchronize()
+        hyb_s += time.perf_counter() - a
+        infos.append(info)
+        e = (ft - ft_ref).abs() / ft_ref.abs().max()
+        errs.append(float(e.max()))
+    P = nx * ny * len(t0s)
+    return {"workload": "learned surrogate: MLP (128 Fourier features, 4x256 GELU, BF16 MFMA, "
+                        "FP32 accumulation) trained on DOP853 samples of the gridded field, "
+                        f"inference on 2701x1001 at {len(t0s)} of the 96 t0 values, T=-72 h, "
+                        "64x64 tiles with 5 probes, fallback when probe error > 0.05 dx",
+            "training_samples": int(inp.shape[0]), "sample_generation_s": t_gen,
+            "training_s": t_train, "training_tflops_bf16": train_tflops,
+            "training_frac_of_bf16_peak": train_tflops / BF16_PEAK_TFLOPS,
+            "loss_first_last": [losses[0], losses[-1]],
+            "plain_fp64_grid_points_per_s": P / plain_s,
+            "surrogate_only_grid_points_per_s": P / mlp_s,
+            "surrogate_only_mlp_tflops": MLP_FLOPS_PER_SAMPLE * P / mlp_s / 1e12,
+            "hybrid_grid_points_per_s": P / hyb_s,
+            "hybrid_speedup_vs_plain": plain_s / hyb_s,
+            "fallback_tiles": int(sum(i["fallback_tiles"] for i in infos)),
+            "tiles": int(sum(i["tiles"] for i in infos)),
+            "probe_err_median_deg": float(np.median([i["probe_err_median"] for i in infos])),
+            "hybrid_ftle_max_rel_vs_plain": float(max(errs))}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is design-only)"}))
+        sys.exit(2)
+    import torch
+
+    torch.cuda.set_device(0)
+    fn = {"gridded": bench_gridded, "double_gyre": bench_double_gyre,
+          "timeseries": bench_timeseries, "precision": bench_precision,
+          "surrogate": bench_surrogate}[args.workload]
+    print(json.dumps(fn(args)))
 
 
 if __name__ == "__main__":

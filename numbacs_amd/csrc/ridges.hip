@@ -100,29 +100,38 @@ __global__ __launch_bounds__(256) void k_sel_hist(const double* __restrict__ f, 
     if (t == 0 && nans) atomicAdd(&st->nan_count, nans);
 }
 
-__global__ void k_sel_pick(SelState* st, int shift) {
-    const int s = threadIdx.x;
-    if (s < 2) {
-        long long k = st->k[s];
-        long long cum = 0;
-        int pick = 255;
-        for (int b = 0; b < 256; ++b) {
-            const long long c = st->hist[s][b];
-            if (cum + c > k) {
-                pick = b;
-                break;
-            }
-            cum += c;
-        }
-        st->k[s] = k - cum;
-        st->prefix[s] |= static_cast<unsigned long long>(pick) << shift;
-        st->mask[s] |= 0xffull << shift;
-    }
+// Pick the digit of both searches: an inclusive scan of the 256-bin
+// histogram in LDS, then the bin whose cumulative count first exceeds k.
+__global__ __launch_bounds__(256) void k_sel_pick(SelState* st, int shift) {
+    __shared__ long long cum[2][256];
+    const int b = threadIdx.x;
+    // read both ranks before any thread updates them (the scan's barriers
+    // order these reads before the single writer below)
+    const long long kk[2] = {st->k[0], st->k[1]};
+    cum[0][b] = st->hist[0][b];
+    cum[1][b] = st->hist[1][b];
     __syncthreads();
-    for (int b = threadIdx.x; b < 256; b += blockDim.x) {
-        st->hist[0][b] = 0;
-        st->hist[1][b] = 0;
+    for (int off = 1; off < 256; off <<= 1) {
+        const long long a0 = b >= off ? cum[0][b - off] : 0;
+        const long long a1 = b >= off ? cum[1][b - off] : 0;
+        __syncthreads();
+        cum[0][b] += a0;
+        cum[1][b] += a1;
+        __syncthreads();
     }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const long long k = kk[s];
+        const long long before = b > 0 ? cum[s][b - 1] : 0;
+        // exactly one bin has before <= k < cum (k < total always holds)
+        if (before <= k && k < cum[s][b]) {
+            st->k[s] = k - before;
+            st->prefix[s] |= static_cast<unsigned long long>(b) << shift;
+            st->mask[s] |= 0xffull << shift;
+        }
+    }
+    st->hist[0][b] = 0;
+    st->hist[1][b] = 0;
 }
 
 // numpy's _lerp: a + (b-a)*g, or b - (b-a)*(1-g) when g >= 0.5

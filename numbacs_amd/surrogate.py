@@ -160,3 +160,106 @@ def grid_inputs(x, y, t0, T):
     n = X.numel()
     return t.stack([X.reshape(n), Y.reshape(n), t.full((n,), float(t0), device=xd.device),
                     t.full((n,), float(T), device=xd.device)], 1).contiguous()
+
+
+# ---------------------------------------------------------------------------
+# Training data from the integrator, training loop, inference with fallback.
+
+def integrate_samples(funcptr, params, requests, rtol=1e-6, atol=1e-8, precision="fp64"):
+    """Integrate tensor-product sample grids with the batched DOP853 launch.
+    requests: list of dict(t0, T, x, y).  Returns (inputs (n, 4) float32,
+    displacements (n, 2) float32) as device tensors, in request order."""
+    from .batch import FTLEBatch
+
+    t = _lib.torch()
+    b = FTLEBatch(funcptr, requests, params, rtol, atol, precision, ftle=False).run()
+    ins, outs = [], []
+    for q, d in zip(requests, b.requests):
+        X, Y = t.meshgrid(d["x"], d["y"], indexing="ij")
+        n = X.numel()
+        x0 = t.stack([X.reshape(n), Y.reshape(n)], 1)
+        fm = d["flowmap"].reshape(n, 2)
+        ins.append(t.cat([x0.float(), t.full((n, 1), float(q["t0"]), device=x0.device),
+                          t.full((n, 1), float(q["T"]), device=x0.device)], 1))
+        outs.append((fm - x0).float())
+    b.close()
+    return t.cat(ins, 0).contiguous(), t.cat(outs, 0).contiguous()
+
+
+def sample_requests(nreq, per_axis, box, t0_range, T_range, seed=0):
+    """Random jittered sample grids: per request a (t0, T) and sorted random
+    x, y axes over the box ((x0, x1), (y0, y1))."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nreq):
+        out.append(dict(t0=float(rng.uniform(*t0_range)), T=float(rng.uniform(*T_range)),
+                        x=np.sort(rng.uniform(box[0][0], box[0][1], per_axis)),
+                        y=np.sort(rng.uniform(box[1][0], box[1][1], per_axis))))
+    return out
+
+
+def train(net, inputs, targets, epochs=3, batch=8192, lr=2e-3, lr_min=1e-4, seed=0,
+          log=None):
+    """Adam over shuffled mini-batches with a cosine learning-rate decay;
+    targets in physical units (divided by net.out_scale here)."""
+    t = _lib.torch()
+    n = int(inputs.shape[0]) // batch * batch
+    tgt = (targets / net.out_scale).contiguous()
+    g = t.Generator(device=inputs.device)
+    g.manual_seed(seed)
+    steps = epochs * (n // batch)
+    k = 0
+    losses = []
+    for ep in range(epochs):
+        perm = t.randperm(int(inputs.shape[0]), device=inputs.device, generator=g)[:n]
+        for s in range(n // batch):
+            idx = perm[s * batch:(s + 1) * batch]
+            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * k / max(steps, 1)))
+            loss = net.train_step(inputs[idx].contiguous(), tgt[idx].contiguous(), lr=cur)
+            k += 1
+            if s % 200 == 0:
+                losses.append(float(loss.item()))
+                if log:
+                    log(f"epoch {ep} step {s} loss {losses[-1]:.3e}")
+    return losses
+
+
+def flowmap_with_fallback(net, funcptr, params, x, y, t0, T, tol, tile=64,
+                          probes=((8, 8), (8, 55), (55, 8), (55, 55), (32, 32)), rtol=1e-6,
+                          atol=1e-8):
+    """Flow map (nx, ny, 2) device tensor from the surrogate, with per-tile
+    probe particles integrated by DOP853 and every tile whose probe
+    displacement error exceeds tol integrated by DOP853 in full.  Returns
+    (flow map, dict of counts and probe errors)."""
+    from .batch import FTLEBatch
+    from .integration import flowmap
+
+    t = _lib.torch()
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    nx, ny = len(x), len(y)
+    inp = grid_inputs(x, y, t0, T)
+    disp = net.predict_device(inp)
+    X, Y = t.meshgrid(_lib.to_dev(x), _lib.to_dev(y), indexing="ij")
+    fm = t.stack([X, Y], -1) + disp.reshape(nx, ny, 2).double()
+    # probes
+    tiles = [(i0, j0) for i0 in range(0, nx, tile) for j0 in range(0, ny, tile)]
+    pidx = []
+    for i0, j0 in tiles:
+        for pi, pj in probes:
+            pidx.append((min(i0 + pi, nx - 1), min(j0 + pj, ny - 1)))
+    pidx = np.array(pidx)
+    pts = np.column_stack([x[pidx[:, 0]], y[pidx[:, 1]]])
+    exact = flowmap(funcptr, t0, T, pts, params, rtol=rtol, atol=atol)
+    pred = _lib.to_host(fm[pidx[:, 0], pidx[:, 1]])
+    err = np.abs(pred - exact).max(1).reshape(len(tiles), len(probes)).max(1)
+    bad = [tiles[k] for k in np.nonzero(err > tol)[0]]
+    if bad:
+        reqs = [dict(t0=t0, T=T, x=x[i0:i0 + tile], y=y[j0:j0 + tile]) for i0, j0 in bad]
+        b = FTLEBatch(funcptr, reqs, params, rtol, atol, ftle=False).run()
+        for (i0, j0), d in zip(bad, b.requests):
+            fm[i0:i0 + d["nx"], j0:j0 + d["ny"]] = d["flowmap"]
+        b.close()
+    info = dict(tiles=len(tiles), fallback_tiles=len(bad), probes=len(pidx),
+                probe_err_max=float(err.max()), probe_err_median=float(np.median(err)))
+    return fm, info

@@ -44,3 +44,32 @@ def test_error_path_without_gpu_work():
     assert b"unknown" in lib.nbx_last_error()
     assert lib.nbx_flow_create_analytic(0, ctypes.byref(out)) == 0
     lib.nbx_flow_destroy(out)
+
+
+def test_batch_and_launch_config_validation_without_gpu():
+    """Argument checks of the batch and launch-config entry points run on
+    the host before any device call."""
+    from numbacs_amd import _lib
+
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    flow = ctypes.c_void_p()
+    assert lib.nbx_flow_create_analytic(0, ctypes.byref(flow)) == 0
+    p = (ctypes.c_double * 6)(-1.0, 0.1, 0.25, 0.0, 0.6283185307179586, 0.0)
+    reqs = (_lib.Request * 1)()
+    assert lib.nbx_batch_create(flow, reqs, 0, p, 6, 1e-6, 1e-8, 100, 0, ctypes.byref(out)) != 0
+    assert b"at least one request" in lib.nbx_last_error()
+    assert lib.nbx_batch_create(flow, reqs, 1, p, 6, 1e-6, 1e-8, 100, 7, ctypes.byref(out)) != 0
+    assert b"precision" in lib.nbx_last_error()
+    assert lib.nbx_batch_create(flow, reqs, 1, p, 3, 1e-6, 1e-8, 100, 0, ctypes.byref(out)) != 0
+    assert b"6 parameters" in lib.nbx_last_error()
+    assert lib.nbx_set_launch_config(2, 0, 3, 64, 0) != 0
+    assert lib.nbx_set_launch_config(0, 0, 2, 64, 0) != 0
+    assert lib.nbx_set_launch_config(2, 0, 2, 96, 0) != 0
+    assert lib.nbx_set_launch_config(2, 0, 2, 64, 65) != 0
+    g, b, q = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert lib.nbx_get_launch_config(2, 1, ctypes.byref(g), ctypes.byref(b), ctypes.byref(q)) == 0
+    assert g.value in (1, 2, 4) and b.value in (64, 128, 256) and q.value >= 0
+    assert lib.nbx_mlp_gemm(0, None, None, 65, 64, 32, None, None, None, None, None) != 0
+    assert b"multiples" in lib.nbx_last_error()
+    lib.nbx_flow_destroy(flow)
