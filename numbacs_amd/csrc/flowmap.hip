@@ -192,7 +192,7 @@ struct PointSource {
 // One solve per group of G lanes (G = 1 for analytic flows); the group's
 // lanes run identical solver arithmetic and split the spline gathers.
 template <class Flow, class R, int G, class Src>
-__global__ __launch_bounds__(256) void k_flowmap(Flow f, Src src, int64_t P, double s0, double s1,
+__global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap(Flow f, Src src, int64_t P, double s0, double s1,
                                                  double rtol, double atol,
                                                  const uint8_t* __restrict__ mask, long long nmax,
                                                  double2* __restrict__ out,

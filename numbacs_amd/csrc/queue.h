@@ -22,6 +22,12 @@
 #include "dop853.h"
 #include "flows.h"
 
+// Minimum wavefronts per SIMD requested from the compiler for the flow-map
+// kernels (caps their VGPR budget at 512 / n registers).
+#ifndef NBX_FLOWMAP_MIN_WAVES
+#define NBX_FLOWMAP_MIN_WAVES 1
+#endif
+
 namespace nbx {
 
 // One flow-map request as the kernel sees it.
@@ -42,7 +48,7 @@ static __global__ void k_queue_reset(unsigned long long* counter) { *counter = 0
 int device_cu_count();
 
 template <class Flow, class R, int G>
-__global__ __launch_bounds__(256) void k_flowmap_queue(
+__global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap_queue(
     Flow f, ReqDev single, const ReqDev* __restrict__ table, const int64_t* __restrict__ offsets,
     int nreq, int64_t total, unsigned long long* __restrict__ counter, double rtol, double atol,
     long long nmax) {

@@ -58,7 +58,8 @@ def ftle_grid_2D(flowmap, T, dx, dy, mask=None):
     nx, ny = int(fm.shape[0]), int(fm.shape[1])
     if nx * ny == 0:
         return np.zeros((nx, ny))
-    out = ftle_grid_2d_device(fm, T, dx, dy, _mask_dev(mask, nx, ny))
+    maskd = _mask_dev(mask, nx, ny)
+    out = ftle_grid_2d_device(fm, T, dx, dy, maskd)
     return _lib.to_host(out)
 
 
@@ -74,9 +75,10 @@ def C_eig_2D(flowmap, dx, dy, mask=None):
         return np.zeros((nx, ny, 2)), np.zeros((nx, ny, 2, 2))
     ev = _lib.empty((nx, ny, 2))
     V = _lib.empty((nx, ny, 2, 2))
+    maskd = _mask_dev(mask, nx, ny)  # held until the kernel has been enqueued and copied back
     _lib.check(_lib.load().nbx_C_eig_2d(_lib.ptr(fm), nx, ny, float(dx), float(dy),
-                                        _lib.ptr(_mask_dev(mask, nx, ny)), _lib.ptr(ev),
-                                        _lib.ptr(V), _lib.stream_ptr()))
+                                        _lib.ptr(maskd), _lib.ptr(ev), _lib.ptr(V),
+                                        _lib.stream_ptr()))
     return _lib.to_host(ev), _lib.to_host(V)
 
 

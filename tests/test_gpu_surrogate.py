@@ -64,8 +64,11 @@ def test_gemm_modes_exact_on_integers(lib, M, N, K):
     Bn = rng.integers(-4, 5, (N, K)).astype(np.float32)
     pre1 = rng.uniform(-3, 3, (M, N)).astype(np.float32)
     D = t.empty((M, N), dtype=t.bfloat16, device="cuda")
-    lib.check(L.nbx_mlp_gemm(1, lib.ptr(Ad), lib.ptr(_bf(t, Bn)), M, N, K, None, lib.ptr(D), None,
-                             lib.ptr(t.from_numpy(pre1).to("cuda")), s))
+    # keep every operand alive until the kernel has run
+    Bnd = _bf(t, Bn)
+    pred = t.from_numpy(pre1).to("cuda")
+    lib.check(L.nbx_mlp_gemm(1, lib.ptr(Ad), lib.ptr(Bnd), M, N, K, None, lib.ptr(D), None,
+                             lib.ptr(pred), s))
     ref = omlp.bf16(omlp.bf16(A @ Bn.T) * omlp.gelu_grad(pre1))
     got = lib.to_host(D.float())
     assert np.mean(got == ref) > 0.99
@@ -80,7 +83,8 @@ def test_gemm_random_bf16(lib):
     A = omlp.bf16(rng.standard_normal((M, K)))
     B = omlp.bf16(rng.standard_normal((K, N)))
     C = t.empty((M, N), dtype=t.float32, device="cuda")
-    lib.check(lib.load().nbx_mlp_gemm(3, lib.ptr(_bf(t, A)), lib.ptr(_bf(t, B)), M, N, K,
+    Ad, Bd = _bf(t, A), _bf(t, B)
+    lib.check(lib.load().nbx_mlp_gemm(3, lib.ptr(Ad), lib.ptr(Bd), M, N, K,
                                       lib.ptr(C), None, None, None, lib.stream_ptr()))
     ref = A.astype(np.float64) @ B.astype(np.float64)
     assert np.abs(lib.to_host(C) - ref).max() < 1e-5 * np.abs(ref).max()
