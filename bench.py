@@ -55,7 +55,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--workload", default="gridded",
-                    choices=["gridded", "double_gyre", "timeseries", "precision", "surrogate"])
+                    choices=["gridded", "double_gyre", "timeseries", "precision", "surrogate",
+                             "copycal"])
     ap.add_argument("--surrogate-samples", type=int, default=4_000_000)
     ap.add_argument("--surrogate-t0s", type=int, default=2)
     return ap.parse_args()
@@ -455,6 +456,21 @@ def bench_surrogate(args):
             "hybrid_ftle_max_rel_vs_plain": float(max(errs))}
 
 
+def bench_copycal(args):
+    """A plain device copy of known size (512 MiB read + 512 MiB written,
+    three dispatches): the reference for scaling rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters to bytes."""
+    import torch
+
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"workload": "device copy calibration", "bytes_per_copy": n * 8, "copies": 3}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -466,7 +482,7 @@ def main():
     torch.cuda.set_device(0)
     fn = {"gridded": bench_gridded, "double_gyre": bench_double_gyre,
           "timeseries": bench_timeseries, "precision": bench_precision,
-          "surrogate": bench_surrogate}[args.workload]
+          "surrogate": bench_surrogate, "copycal": bench_copycal}[args.workload]
     print(json.dumps(fn(args)))
 
 
