@@ -220,27 +220,53 @@ struct Spline3 {
         bspline_weights(s1, wx);
         bspline_weights(s2, wy);
         const P* base = C + (static_cast<long>(it) * st + static_cast<long>(ix) * sx + iy);
-        const int g = G > 1 ? static_cast<int>(__lane_id() & (G - 1)) : 0;
+        // explicit fma: the accumulation order and rounding do not depend on
+        // how the compiler contracts each kernel instantiation
         R au = 0, av = 0;
+        if constexpr (G == 1) {
 #pragma unroll
-        for (int aa = 0; aa < 4 / G; ++aa) {
-            const int a = aa * G + g;
-            const R wa = G == 1 ? wt[aa] : pick4(wt, a);
+            for (int a = 0; a < 4; ++a) {
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const P* row = base + (a * st + b * sx);
-                const P c0 = row[0], c1 = row[1], c2 = row[2], c3 = row[3];
-                R ru = wy[0] * c0.x;
-                R rv = wy[0] * c0.y;
-                ru += wy[1] * c1.x;
-                rv += wy[1] * c1.y;
-                ru += wy[2] * c2.x;
-                rv += wy[2] * c2.y;
-                ru += wy[3] * c3.x;
-                rv += wy[3] * c3.y;
-                const R wab = wa * wx[b];
-                au += wab * ru;
-                av += wab * rv;
+                for (int b = 0; b < 4; ++b) {
+                    const P* row = base + (a * st + b * sx);
+                    const P c0 = row[0], c1 = row[1], c2 = row[2], c3 = row[3];
+                    R ru = wy[0] * c0.x;
+                    R rv = wy[0] * c0.y;
+                    ru = fma(wy[1], c1.x, ru);
+                    rv = fma(wy[1], c1.y, rv);
+                    ru = fma(wy[2], c2.x, ru);
+                    rv = fma(wy[2], c2.y, rv);
+                    ru = fma(wy[3], c3.x, ru);
+                    rv = fma(wy[3], c3.y, rv);
+                    const R wab = wt[a] * wx[b];
+                    au = fma(wab, ru, au);
+                    av = fma(wab, rv, av);
+                }
+            }
+        } else {
+            // lane g of the group takes the t-slices a = g, g+G, ...: each
+            // lane reads whole 4-pair rows (64 contiguous bytes in FP64)
+            const int g = static_cast<int>(__lane_id() & (G - 1));
+#pragma unroll
+            for (int aa = 0; aa < 4 / G; ++aa) {
+                const int a = aa * G + g;
+                const R wa = pick4(wt, a);
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const P* row = base + (a * st + b * sx);
+                    const P c0 = row[0], c1 = row[1], c2 = row[2], c3 = row[3];
+                    R ru = wy[0] * c0.x;
+                    R rv = wy[0] * c0.y;
+                    ru = fma(wy[1], c1.x, ru);
+                    rv = fma(wy[1], c1.y, rv);
+                    ru = fma(wy[2], c2.x, ru);
+                    rv = fma(wy[2], c2.y, rv);
+                    ru = fma(wy[3], c3.x, ru);
+                    rv = fma(wy[3], c3.y, rv);
+                    const R wab = wa * wx[b];
+                    au = fma(wab, ru, au);
+                    av = fma(wab, rv, av);
+                }
             }
         }
         su = group_sum<G>(au);
