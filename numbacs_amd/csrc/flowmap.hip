@@ -234,11 +234,11 @@ __global__ void k_flow_eval(Flow f, const double* __restrict__ t, const double2*
 }
 
 // defaults: the best shapes measured on the gridded-velocity and
-// double-gyre workloads (2 lanes per particle and 64-thread blocks for
+// double-gyre workloads (4 lanes per particle and 64-thread blocks for
 // spline flows); the autotuner overrides them per workload
 LaunchCfg g_cfg[3][2] = {{{1, 256, 0}, {1, 256, 0}},
                          {{1, 256, 0}, {1, 256, 0}},
-                         {{2, 64, 0}, {2, 64, 0}}};
+                         {{4, 64, 0}, {4, 64, 0}}};
 std::mutex g_cfg_mutex;
 
 LaunchCfg launch_cfg(int kind, int precision) {

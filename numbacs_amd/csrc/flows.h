@@ -100,22 +100,26 @@ struct BickleyJet {
 };
 
 // Uniform cubic B-spline basis at local coordinate s in [0, 1].
+// The 1/6 scaling is a multiplication by the rounded reciprocal (within one
+// rounding of the quotient), which spares twelve IEEE divisions per
+// evaluation; the 1/2 of the derivative weights is exact either way.
 template <class R>
 __device__ __forceinline__ void bspline_weights(R s, R w[4]) {
+    constexpr R sixth = R(1) / R(6);
     const R s2 = s * s, s3 = s2 * s, u = R(1) - s;
-    w[0] = u * u * u / R(6);
-    w[1] = (R(3) * s3 - R(6) * s2 + R(4)) / R(6);
-    w[2] = (R(-3) * s3 + R(3) * s2 + R(3) * s + R(1)) / R(6);
-    w[3] = s3 / R(6);
+    w[0] = u * u * u * sixth;
+    w[1] = (R(3) * s3 - R(6) * s2 + R(4)) * sixth;
+    w[2] = (R(-3) * s3 + R(3) * s2 + R(3) * s + R(1)) * sixth;
+    w[3] = s3 * sixth;
 }
 
 template <class R>
 __device__ __forceinline__ void bspline_dweights(R s, R dw[4]) {
     const R s2 = s * s, u = R(1) - s;
-    dw[0] = -u * u / R(2);
-    dw[1] = (R(3) * s2 - R(4) * s) / R(2);
-    dw[2] = (R(-3) * s2 + R(2) * s + R(1)) / R(2);
-    dw[3] = s2 / R(2);
+    dw[0] = -u * u * R(0.5);
+    dw[1] = (R(3) * s2 - R(4) * s) * R(0.5);
+    dw[2] = (R(-3) * s2 + R(2) * s + R(1)) * R(0.5);
+    dw[3] = s2 * R(0.5);
 }
 
 // w[i] for a lane-dependent i without dynamic register indexing
@@ -126,7 +130,9 @@ __device__ __forceinline__ R pick4(const R w[4], int i) {
 
 // Python's float modulo (result takes the sign of the divisor).
 __device__ __forceinline__ double py_mod(double a, double m) {
-    double r = fmod(a, m);
+    // fmod(a, m) == a exactly when |a| < |m|: the library call only runs for
+    // arguments a full period or more away
+    double r = fabs(a) < fabs(m) ? a : fmod(a, m);
     if (r != 0.0) {
         if ((m < 0) != (r < 0)) r += m;
     } else {
@@ -144,9 +150,13 @@ template <> struct CoefPair<float> { using type = float2; };
 // Sum of a value over the G lanes of an aligned lane group, with a butterfly
 // of DPP quad permutations, so every lane of the group ends with the same
 // bits (the additions are commutative pairwise, and each level combines the
-// same two partial sums in every lane).
+// same two partial sums in every lane).  The empty asm materialises v first:
+// otherwise the compiler may fuse the multiply that produced v into the
+// first add (fma(a, b, other lane's rounded a*b)), and the lanes of a group
+// would then round differently and take different step-size decisions.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
+    asm volatile("" : "+v"(v));
     if constexpr (G >= 2)
         v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
     if constexpr (G >= 4)
@@ -169,6 +179,7 @@ __device__ __forceinline__ double group_sum(double v) {
         return __longlong_as_double((static_cast<long long>(hi) << 32) |
                                     static_cast<unsigned int>(lo));
     };
+    asm volatile("" : "+v"(v));
     if constexpr (G >= 2) v += xchg(v, 0xB1);
     if constexpr (G >= 4) v += xchg(v, 0x4E);
     return v;
@@ -179,9 +190,13 @@ __device__ __forceinline__ double group_sum(double v) {
 // C holds (nt+2, nx+2, ny+2) (u, v) pairs, y fastest.  R is the coefficient
 // and evaluation precision (double for the plain path, float for the
 // FP32-displacement form).  G lanes (1, 2 or 4, aligned in the wavefront)
-// share one evaluation: lane g of the group gathers the t-slices a with
-// a % G == g (16 pairs each for G = 4) and the partial sums are combined
-// with group_sum, which multiplies the loads in flight per particle by G.
+// share one evaluation and combine partial sums with group_sum:
+//   G = 2: lane g gathers the t-slices a with a % 2 == g (whole 4-pair rows);
+//   G = 4: lane g gathers y-coefficient g of all 16 rows, so one load
+//          instruction covers one contiguous row per particle -- a third of
+//          the distinct cache lines per instruction of the row-per-lane
+//          forms, and the vector L1's tag rate (one line per clock) is what
+//          bounds these gathers.
 //
 // Out-of-box queries (parity unpinned: the reference's dependency is not
 // available): extrap 0/2 clamp the query into the box per axis (value of the
@@ -243,9 +258,31 @@ struct Spline3 {
                     av = fma(wab, rv, av);
                 }
             }
+        } else if constexpr (G == 4) {
+            // lane g of the group takes y-coefficient g of all 16 rows, so
+            // the group's four lanes read each 4-pair row in one load; the
+            // y-weight is applied last, once per lane
+            const int g = static_cast<int>(__lane_id() & 3);
+            const P* lbase = base + g;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                R ru = 0, rv = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const P c = lbase[a * st + b * sx];
+                    ru = fma(wx[b], c.x, ru);
+                    rv = fma(wx[b], c.y, rv);
+                }
+                au = fma(wt[a], ru, au);
+                av = fma(wt[a], rv, av);
+            }
+            const R wc = pick4(wy, g);
+            au *= wc;
+            av *= wc;
         } else {
-            // lane g of the group takes the t-slices a = g, g+G, ...: each
-            // lane reads whole 4-pair rows (64 contiguous bytes in FP64)
+            static_assert(G == 2, "lane groups of 1, 2 or 4");
+            // lane g of the group takes the t-slices a = g, g+2: each lane
+            // reads whole 4-pair rows (64 contiguous bytes in FP64)
             const int g = static_cast<int>(__lane_id() & (G - 1));
 #pragma unroll
             for (int aa = 0; aa < 4 / G; ++aa) {
@@ -347,14 +384,16 @@ struct Spline3 {
     __device__ __forceinline__ void operator()(double t, double y0, double y1, R& f0,
                                                R& f1) const {
         const double tt = p0 * t;
+        // one inlined spline evaluation per call for every coordinate mode
+        double xx = y0;
+        if (spherical == 1) xx = py_mod(y0 - 180, 360) - 180;
+        else if (spherical != 0) xx = py_mod(y0, 360);
         R su, sv;
+        eval(tt, xx, y1, su, sv);
         if (spherical == 0) {
-            eval(tt, y0, y1, su, sv);
             f0 = static_cast<R>(p0 * static_cast<double>(su));
             f1 = static_cast<R>(p0 * static_cast<double>(sv));
         } else {
-            const double xx = spherical == 1 ? py_mod(y0 - 180, 360) - 180 : py_mod(y0, 360);
-            eval(tt, xx, y1, su, sv);
             f0 = static_cast<R>(((p0 * static_cast<double>(su)) * 180) /
                                 (kPi * r * cos(y1 * kPi / 180)));
             f1 = static_cast<R>(((p0 * static_cast<double>(sv)) * 180) / (kPi * r));
