@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--launch", default=None, metavar="LANES,BLOCK,PERSIST",
+                    help="gridded workload: fixed flow-map launch shape instead of the "
+                         "autotuner (e.g. 4,128,0)")
     ap.add_argument("--workload", default="gridded",
                     choices=["gridded", "double_gyre", "timeseries", "precision", "surrogate",
                              "copycal"])
@@ -114,8 +117,14 @@ def bench_gridded(args):
         flowmap_grid_2d_device(W["flow"], t0, T, xd, yd, params, 1e-6, 1e-8, out=fm,
                                status=status, nsteps=nsteps)
 
-    tuned, tune_ms = (autotune.get(2, 0), None) if args.no_tune else autotune.tune(
-        _lib.NBX_FLOW_SPLINE3, _lib.NBX_FP64, flowmap, nx * ny, use_cache=False)
+    if args.launch:
+        autotune.set_config(_lib.NBX_FLOW_SPLINE3, _lib.NBX_FP64,
+                            tuple(int(v) for v in args.launch.split(",")))
+    if args.launch or args.no_tune:
+        tuned, tune_ms = autotune.get(_lib.NBX_FLOW_SPLINE3, _lib.NBX_FP64), None
+    else:
+        tuned, tune_ms = autotune.tune(_lib.NBX_FLOW_SPLINE3, _lib.NBX_FP64, flowmap, nx * ny,
+                                       use_cache=False)
 
     ev_fm = []
 
@@ -400,12 +409,16 @@ def bench_surrogate(args):
     inp, tgt = S.integrate_samples(W["flow"], [-1.0], reqs)
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t_gen
+    print(f"surrogate: {int(inp.shape[0])} DOP853 samples in {t_gen:.1f} s", file=sys.stderr,
+          flush=True)
     net = S.MLPSurrogate([-32.5, 20.0, 83.5, -72.0], [67.5, 25.0, 11.5, 1.0], [15.0, 15.0],
                          sigma=2.0, seed=7)
     t_train = time.perf_counter()
     losses = S.train(net, inp, tgt, epochs=3, batch=8192, lr=2e-3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t_train
+    print(f"surrogate: trained in {t_train:.1f} s, loss {losses[0]:.4g} -> {losses[-1]:.4g}",
+          file=sys.stderr, flush=True)
     steps = 3 * (int(inp.shape[0]) // 8192)
     train_tflops = 3 * MLP_FLOPS_PER_SAMPLE * 8192 * steps / t_train / 1e12
 
@@ -436,6 +449,8 @@ def bench_surrogate(args):
         infos.append(info)
         e = (ft - ft_ref).abs() / ft_ref.abs().max()
         errs.append(float(e.max()))
+        print(f"surrogate: t0={t0:g} fallback {info['fallback_tiles']}/{info['tiles']} tiles",
+              file=sys.stderr, flush=True)
     P = nx * ny * len(t0s)
     return {"workload": "learned surrogate: MLP (128 Fourier features, 4x256 GELU, BF16 MFMA, "
                         "FP32 accumulation) trained on DOP853 samples of the gridded field, "

@@ -12,7 +12,7 @@ struct nbx_flow {
     int kind;
     // spline flows
     double2* coef = nullptr;  // (nt+2, nx+2, ny+2) interleaved (u, v), device
-    float2* coef32 = nullptr; // FP32 copy for the FP32-displacement path
+    float4* coef32 = nullptr; // FP32 copy, time-paired, for the FP32-displacement path
     int64_t ncoef = 0;
     double grid9[9] = {0};
     int spherical = 0;

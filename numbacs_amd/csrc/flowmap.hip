@@ -155,13 +155,18 @@ __global__ void k_prefilter_axis(double* __restrict__ C, int nt, int nx, int ny,
     prefilter_line(p, stride, n);
 }
 
+// (u, v) pairs in FP64, and FP32 elements pairing time index t with t + 1
+// (slab = one time slice of coefficients; the last slice repeats itself).
 __global__ void k_interleave(const double* __restrict__ u, const double* __restrict__ v, int64_t n,
-                             double2* __restrict__ out, float2* __restrict__ out32) {
+                             int64_t slab, double2* __restrict__ out,
+                             float4* __restrict__ out32) {
     for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
          k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const double a = u[k], b = v[k];
+        const int64_t k1 = k + slab < n ? k + slab : k;
         out[k] = make_double2(a, b);
-        out32[k] = make_float2(static_cast<float>(a), static_cast<float>(b));
+        out32[k] = make_float4(static_cast<float>(a), static_cast<float>(b),
+                               static_cast<float>(u[k1]), static_cast<float>(v[k1]));
     }
 }
 
@@ -234,11 +239,11 @@ __global__ void k_flow_eval(Flow f, const double* __restrict__ t, const double2*
 }
 
 // defaults: the best shapes measured on the gridded-velocity and
-// double-gyre workloads (4 lanes per particle and 64-thread blocks for
-// spline flows); the autotuner overrides them per workload
+// double-gyre workloads (spline flows: 4 lanes per particle, 128-thread
+// blocks); the autotuner overrides them per workload
 LaunchCfg g_cfg[3][2] = {{{1, 256, 0}, {1, 256, 0}},
                          {{1, 256, 0}, {1, 256, 0}},
-                         {{4, 64, 0}, {4, 64, 0}}};
+                         {{4, 128, 0}, {4, 128, 0}}};
 std::mutex g_cfg_mutex;
 
 LaunchCfg launch_cfg(int kind, int precision) {
@@ -382,14 +387,14 @@ int nbx_flow_create_spline3(const double* grid9_host, const double* Cu, const do
     fl->r = r;
     fl->ncoef = ncoef;
     if (hipMalloc(&fl->coef, sizeof(double2) * ncoef) != hipSuccess ||
-        hipMalloc(&fl->coef32, sizeof(float2) * ncoef) != hipSuccess) {
+        hipMalloc(&fl->coef32, sizeof(float4) * ncoef) != hipSuccess) {
         nbx_flow_destroy(fl);
         set_error("hipMalloc of spline coefficients failed");
         return -1;
     }
     hipStream_t s = as_stream(stream);
     k_interleave<<<blocks_for(ncoef, 256) < 4096 ? blocks_for(ncoef, 256) : 4096, 256, 0, s>>>(
-        Cu, Cv, ncoef, fl->coef, fl->coef32);
+        Cu, Cv, ncoef, static_cast<int64_t>(n[1] + 2) * (n[2] + 2), fl->coef, fl->coef32);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         nbx_flow_destroy(fl);

@@ -141,11 +141,14 @@ __device__ __forceinline__ double py_mod(double a, double m) {
     return r;
 }
 
-// Coefficient storage element: (u, v) interleaved so one gather serves both
-// velocity components.
+// Coefficient storage element.  FP64: one (u, v) pair, so one gather serves
+// both velocity components.  FP32: (u, v) at time index t and at t + 1 in one
+// 16-byte element, so one gather serves two time slices of the stencil and a
+// stencil is 8 rows of 4 elements instead of 16 rows (the gathers are bound
+// by distinct cache lines, not bytes).
 template <class R> struct CoefPair;
 template <> struct CoefPair<double> { using type = double2; };
-template <> struct CoefPair<float> { using type = float2; };
+template <> struct CoefPair<float> { using type = float4; };
 
 // Sum of a value over the G lanes of an aligned lane group, with a butterfly
 // of DPP quad permutations, so every lane of the group ends with the same
@@ -222,6 +225,35 @@ struct Spline3 {
         s = static_cast<R>(u - k);
     }
 
+    // (u, v) of stencil time slice a (0..3), x row b, y column c, relative
+    // to the stencil origin base
+    __device__ __forceinline__ void coef(const P* base, int a, int b, int c, R& u, R& v) const {
+        if constexpr (sizeof(R) == sizeof(double)) {
+            const P e = base[a * st + b * sx + c];
+            u = e.x;
+            v = e.y;
+        } else {
+            const P e = base[(a & ~1) * st + b * sx + c];
+            u = (a & 1) ? e.z : e.x;
+            v = (a & 1) ? e.w : e.y;
+        }
+    }
+
+    // y-weighted sum of one stencil row (slice a, x row b)
+    __device__ __forceinline__ void row_sum(const P* base, int a, int b, const R wy[4], R& ru,
+                                            R& rv) const {
+        R cu, cv;
+        coef(base, a, b, 0, cu, cv);
+        ru = wy[0] * cu;
+        rv = wy[0] * cv;
+#pragma unroll
+        for (int c = 1; c < 4; ++c) {
+            coef(base, a, b, c, cu, cv);
+            ru = fma(wy[c], cu, ru);
+            rv = fma(wy[c], cv, rv);
+        }
+    }
+
     // (u, v) at an in-box point
     __device__ __forceinline__ void eval_in(double qt, double qx, double qy, R& su,
                                             R& sv) const {
@@ -243,16 +275,8 @@ struct Spline3 {
             for (int a = 0; a < 4; ++a) {
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const P* row = base + (a * st + b * sx);
-                    const P c0 = row[0], c1 = row[1], c2 = row[2], c3 = row[3];
-                    R ru = wy[0] * c0.x;
-                    R rv = wy[0] * c0.y;
-                    ru = fma(wy[1], c1.x, ru);
-                    rv = fma(wy[1], c1.y, rv);
-                    ru = fma(wy[2], c2.x, ru);
-                    rv = fma(wy[2], c2.y, rv);
-                    ru = fma(wy[3], c3.x, ru);
-                    rv = fma(wy[3], c3.y, rv);
+                    R ru, rv;
+                    row_sum(base, a, b, wy, ru, rv);
                     const R wab = wt[a] * wx[b];
                     au = fma(wab, ru, au);
                     av = fma(wab, rv, av);
@@ -269,9 +293,10 @@ struct Spline3 {
                 R ru = 0, rv = 0;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const P c = lbase[a * st + b * sx];
-                    ru = fma(wx[b], c.x, ru);
-                    rv = fma(wx[b], c.y, rv);
+                    R cu, cv;
+                    coef(lbase, a, b, 0, cu, cv);
+                    ru = fma(wx[b], cu, ru);
+                    rv = fma(wx[b], cv, rv);
                 }
                 au = fma(wt[a], ru, au);
                 av = fma(wt[a], rv, av);
@@ -281,25 +306,17 @@ struct Spline3 {
             av *= wc;
         } else {
             static_assert(G == 2, "lane groups of 1, 2 or 4");
-            // lane g of the group takes the t-slices a = g, g+2: each lane
-            // reads whole 4-pair rows (64 contiguous bytes in FP64)
+            // lane g of the group takes the t-slices a = 2g, 2g+1 (one element
+            // row pair in FP32): each lane reads whole 4-element rows
             const int g = static_cast<int>(__lane_id() & (G - 1));
 #pragma unroll
-            for (int aa = 0; aa < 4 / G; ++aa) {
-                const int a = aa * G + g;
+            for (int aa = 0; aa < 2; ++aa) {
+                const int a = 2 * g + aa;
                 const R wa = pick4(wt, a);
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const P* row = base + (a * st + b * sx);
-                    const P c0 = row[0], c1 = row[1], c2 = row[2], c3 = row[3];
-                    R ru = wy[0] * c0.x;
-                    R rv = wy[0] * c0.y;
-                    ru = fma(wy[1], c1.x, ru);
-                    rv = fma(wy[1], c1.y, rv);
-                    ru = fma(wy[2], c2.x, ru);
-                    rv = fma(wy[2], c2.y, rv);
-                    ru = fma(wy[3], c3.x, ru);
-                    rv = fma(wy[3], c3.y, rv);
+                    R ru, rv;
+                    row_sum(base, a, b, wy, ru, rv);
                     const R wab = wa * wx[b];
                     au = fma(wab, ru, au);
                     av = fma(wab, rv, av);
@@ -335,15 +352,15 @@ struct Spline3 {
         for (int a = 0; a < 4; ++a) {
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const P* row = base + (a * st + b * sx);
                 R ru = 0, rv = 0, du = 0, dv = 0;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const P cc = row[c];
-                    ru += wy[c] * cc.x;
-                    rv += wy[c] * cc.y;
-                    du += dy[c] * cc.x;
-                    dv += dy[c] * cc.y;
+                    R cu, cv;
+                    coef(base, a, b, c, cu, cv);
+                    ru += wy[c] * cu;
+                    rv += wy[c] * cv;
+                    du += dy[c] * cu;
+                    dv += dy[c] * cv;
                 }
                 au += wt[a] * wx[b] * ru;
                 av += wt[a] * wx[b] * rv;
