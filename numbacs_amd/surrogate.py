@@ -7,7 +7,7 @@ accumulation (numbacs_amd/csrc/mlp.hip); every kernel call goes through the
 C ABI.  At inference the particle grid is cut into 64x64 tiles; a few probe
 particles per tile are integrated with DOP853 and any tile whose probe
 displacement error exceeds the threshold is integrated with DOP853 in full,
-so the integrator stays both the fallback and the judge of the surrogate.
+so the integrator stays both the fallback and the arbiter of the surrogate.
 
 Network (defaults): 128 Fourier features (64 frequencies), 4 hidden layers
 of 256 with tanh-form GELU, a 2-output linear head.  The numpy restatement

@@ -1,7 +1,7 @@
 """CPU oracle of the flow-map surrogate MLP -- TEST INFRASTRUCTURE ONLY.
 
 numpy restatement of the network the HIP kernels implement (no reference
-counterpart: the surrogate is a north-star addition with no NumbaCS code):
+counterpart: NumbaCS has no learned surrogate):
 
   u   = (inputs - shift) / scale                       (x0, y0, t0, T), fp32
   z   = 2*pi * u @ B                                    B: (4, F/2) fp32
