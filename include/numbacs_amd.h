@@ -95,6 +95,30 @@ int nbx_flowmap_pts(const nbx_flow* flow, double t0, double T, const double* pts
                     const uint8_t* mask, int64_t max_steps, int precision, double* out,
                     int32_t* status, int32_t* nsteps, nbx_stream stream);
 
+/* flowmap_aux_grid_2D (integration.py:249-464): n_aux = 5 particles per grid
+ * point (eig_main != 0: offsets (h,0), (-h,0), (0,h), (0,-h), (0,0)) or the
+ * first 4; out (nx, ny, n_aux, 2).  compute_edge != 0: edge points integrate
+ * only the centre particle (eig_main) or all 4; compute_edge == 0: edge
+ * points are left out.  Left-out and masked particles (mask (nx*ny)) get 0
+ * in out and status 0; status / nsteps hold nx*ny*n_aux entries. */
+int nbx_flowmap_aux_grid_2d(const nbx_flow* flow, double t0, double T, const double* x, int nx,
+                            const double* y, int ny, double h, int eig_main, int compute_edge,
+                            const double* params_host, int nparams, double rtol, double atol,
+                            const uint8_t* mask, int64_t max_steps, int precision, double* out,
+                            int32_t* status, int32_t* nsteps, nbx_stream stream);
+
+/* flowmap_composition (integration.py:609-644): out (nx, ny, 2) = the chain
+ * of nT flow maps on the uniform grid [x0,x1] x [y0,y1] (nx x ny), each
+ * sampled bilinearly at the previous one's image, 0 outside the closed box
+ * (eval_linear with CONSTANT extrapolation).  flowmaps holds nslots
+ * (nx, ny, 2) slots; map k of the chain is slot (first + k) % nslots (the
+ * reference's flowmaps[k] with first = 0, nslots = nT).  For nT = 1 the
+ * single map is applied twice, as the reference's loop bounds do.  Used by
+ * flowmap_composition_initial / _step (integration.py:647-737). */
+int nbx_flowmap_compose(const double* flowmaps, int nslots, int first, int nT, double x0,
+                        double x1, int nx, double y0, double y1, int ny, double* out,
+                        nbx_stream stream);
+
 /* Launch shape of the flow-map kernels for one (flow kind, precision):
  * lanes_per_particle (1, 2 or 4; spline flows only: the lanes of a group
  * split each tricubic gather), threads per block (64, 128 or 256), and
@@ -167,6 +191,22 @@ int nbx_C_eig_2d(const double* fm, int nx, int ny, double dx, double dy, const u
  * in one pass; writes ftle (nx,ny) and eigvec_max (nx,ny,2). */
 int nbx_ftle_eig_2d(const double* fm, int nx, int ny, double T, double dx, double dy,
                     const uint8_t* mask, double* ftle, double* eigvec_max, nbx_stream stream);
+
+/* C_tensor_2D (diagnostics.py:67-110): C (nx, ny, 3) = (C11, C12, C22) of the
+ * aux-grid gradient (gradF_aux_stencil_2D, utils.py:50-84) on i, j in
+ * [2, n-3]; zeros elsewhere and at masked points.  fm_aux (nx, ny, naux, 2),
+ * naux 4 or 5. */
+int nbx_C_tensor_aux_2d(const double* fm_aux, int nx, int ny, int naux, double h,
+                        const uint8_t* mask, double* C, nbx_stream stream);
+
+/* C_eig_aux_2D (diagnostics.py:113-197): eig_main != 0 (naux = 5): on i, j in
+ * [2, n-3] eigenvectors of the aux-grid tensor and eigenvalues of the
+ * main-grid tensor (gradF_main_stencil_2D, utils.py:88-124); eig_main == 0:
+ * both from the aux-grid tensor on i, j in [1, n-2].  Layouts as
+ * nbx_C_eig_2d. */
+int nbx_C_eig_aux_2d(const double* fm_aux, int nx, int ny, int naux, double dx, double dy,
+                     double h, int eig_main, const uint8_t* mask, double* eigvals,
+                     double* eigvecs, nbx_stream stream);
 
 /* --- ridges ------------------------------------------------------------ */
 

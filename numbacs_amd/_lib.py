@@ -40,6 +40,11 @@ SIGNATURES = {
                                    _c_d, _c_p, _c_i64, _c_i, _c_p, _c_p, _c_p, _c_p]),
     "nbx_flowmap_pts": (_c_i, [_c_p, _c_d, _c_d, _c_p, _c_i64, _c_p, _c_i, _c_d, _c_d, _c_p,
                                _c_i64, _c_i, _c_p, _c_p, _c_p, _c_p]),
+    "nbx_flowmap_aux_grid_2d": (_c_i, [_c_p, _c_d, _c_d, _c_p, _c_i, _c_p, _c_i, _c_d, _c_i,
+                                       _c_i, _c_p, _c_i, _c_d, _c_d, _c_p, _c_i64, _c_i, _c_p,
+                                       _c_p, _c_p, _c_p]),
+    "nbx_flowmap_compose": (_c_i, [_c_p, _c_i, _c_i, _c_i, _c_d, _c_d, _c_i, _c_d, _c_d, _c_i,
+                                   _c_p, _c_p]),
     "nbx_set_launch_config": (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i]),
     "nbx_get_launch_config": (_c_i, [_c_i, _c_i, _c_p, _c_p, _c_p]),
     "nbx_batch_create": (_c_i, [_c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_i64, _c_i, _c_p]),
@@ -60,6 +65,9 @@ SIGNATURES = {
     "nbx_mlp_to_bf16": (_c_i, [_c_p, _c_i64, _c_p, _c_p]),
     "nbx_ftle_grid_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_d, _c_p, _c_p, _c_p]),
     "nbx_C_eig_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_p, _c_p, _c_p, _c_p]),
+    "nbx_C_tensor_aux_2d": (_c_i, [_c_p, _c_i, _c_i, _c_i, _c_d, _c_p, _c_p, _c_p]),
+    "nbx_C_eig_aux_2d": (_c_i, [_c_p, _c_i, _c_i, _c_i, _c_d, _c_d, _c_d, _c_i, _c_p, _c_p,
+                                _c_p, _c_p]),
     "nbx_ftle_eig_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_d, _c_p, _c_p, _c_p, _c_p]),
     "nbx_percentile": (_c_i, [_c_p, _c_i64, _c_d, _c_p, _c_p]),
     "nbx_ridge_pts": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_d, _c_d, _c_p,

@@ -19,8 +19,9 @@ PRECISION = {"fp64": _lib.NBX_FP64, "fp32": _lib.NBX_FP32_DISP}
 class FTLEBatch:
     """A set of flow-map + FTLE requests over one flow.
 
-    requests: iterable of dicts with keys t0, T, x, y and optionally mask;
-    x, y are the particle axes (host arrays or device tensors).  All
+    requests: iterable of dicts with keys t0, T, x, y and optionally mask and
+    flowmap (a device tensor the flow map is written to); x, y are the
+    particle axes (host arrays or device tensors).  All
     requests share params, tolerances and precision ('fp64' or 'fp32', the
     FP32-displacement form).  With eigvec=True each request's FTLE comes from
     the eigen-solver together with eigvec_max (the ridge path).
@@ -44,7 +45,11 @@ class FTLEBatch:
             d = dict(t0=float(q["t0"]), T=float(q["T"]), x=x, y=y, nx=nx, ny=ny)
             d["mask"] = None if q.get("mask") is None else _lib.to_dev(
                 np.asarray(q["mask"]).astype(np.uint8), np.uint8)
-            d["flowmap"] = _lib.empty((nx, ny, 2))
+            out = q.get("flowmap")
+            if out is not None and (tuple(out.shape) != (nx, ny, 2) or not out.is_contiguous()):
+                raise ValueError("a request's flowmap output must be a contiguous (nx, ny, 2) "
+                                 "float64 device tensor")
+            d["flowmap"] = _lib.empty((nx, ny, 2)) if out is None else out
             d["status"] = _lib.empty((nx, ny), "i32")
             d["nsteps"] = _lib.empty((nx, ny), "i32")
             d["ftle"] = _lib.empty((nx, ny)) if ftle else None

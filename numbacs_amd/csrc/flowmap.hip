@@ -174,24 +174,57 @@ __global__ void k_interleave(const double* __restrict__ u, const double* __restr
 // Flow-map kernels.
 
 // Particle sources: a tensor grid x[i], y[j] in 'ij' order, or a point list.
+// pos() returns false for a particle the source itself leaves out (its
+// outputs are written as for a masked one); cell() is its mask index.
 struct GridSource {
     const double* __restrict__ xs;
     const double* __restrict__ ys;
     int ny;
-    __device__ __forceinline__ void pos(int64_t p, double& px, double& py) const {
+    __device__ __forceinline__ bool pos(int64_t p, double& px, double& py) const {
         const int64_t i = p / ny;
         px = xs[i];
         py = ys[p - i * ny];
+        return true;
     }
+    __device__ __forceinline__ int64_t cell(int64_t p) const { return p; }
 };
 
 struct PointSource {
     const double2* __restrict__ pts;
-    __device__ __forceinline__ void pos(int64_t p, double& px, double& py) const {
+    __device__ __forceinline__ bool pos(int64_t p, double& px, double& py) const {
         const double2 q = pts[p];
         px = q.x;
         py = q.y;
+        return true;
     }
+    __device__ __forceinline__ int64_t cell(int64_t p) const { return p; }
+};
+
+// The auxiliary grid of flowmap_aux_grid_2D (integration.py:249-464):
+// particle p = (i*ny + j)*naux + k starts at (x[i], y[j]) + offset k, the
+// offsets being (h,0), (-h,0), (0,h), (0,-h) and, with eig_main, (0,0).
+// On edge points only the centre particle runs (eig_main) or all four (not
+// eig_main); without compute_edge edge points are left out.
+struct AuxSource {
+    const double* __restrict__ xs;
+    const double* __restrict__ ys;
+    int nx, ny, naux;
+    double h;
+    bool eig_main, compute_edge;
+    __device__ __forceinline__ bool pos(int64_t p, double& px, double& py) const {
+        const int64_t c = p / naux;
+        const int k = static_cast<int>(p - c * naux);
+        const int64_t i = c / ny;
+        const int64_t j = c - i * ny;
+        const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+        if (edge && (!compute_edge || (eig_main && k != 4))) return false;
+        const double ox = k == 0 ? h : (k == 1 ? -h : 0.0);
+        const double oy = k == 2 ? h : (k == 3 ? -h : 0.0);
+        px = xs[i] + ox;
+        py = ys[j] + oy;
+        return true;
+    }
+    __device__ __forceinline__ int64_t cell(int64_t p) const { return p / naux; }
 };
 
 // One solve per group of G lanes (G = 1 for analytic flows); the group's
@@ -207,7 +240,9 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap(Flow f, 
     const int64_t p = tid / G;
     if (p >= P) return;  // whole groups leave together
     const bool lead = (tid & (G - 1)) == 0;
-    if (mask != nullptr && mask[p]) {
+    double px, py;
+    const bool run = src.pos(p, px, py);
+    if (!run || (mask != nullptr && mask[src.cell(p)])) {
         if (lead) {
             out[p] = make_double2(0.0, 0.0);
             if (status) status[p] = NBX_ST_MASKED;
@@ -215,8 +250,6 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap(Flow f, 
         }
         return;
     }
-    double px, py;
-    src.pos(p, px, py);
     Dop853State<R> s;
     dop853_init(f, s, s0, s1, px, py, rtol, atol);
     while (dop853_step(f, s, rtol, atol, nmax)) {
@@ -480,6 +513,23 @@ int nbx_flowmap_grid_2d(const nbx_flow* fl, double t0, double T, const double* x
     SolveArgs a{P, params[0] * t0, params[0] * (t0 + T), rtol, atol, mask, max_steps,
                 reinterpret_cast<double2*>(out), status, nsteps};
     return solve(fl, params, np, GridSource{x, y, ny}, a, precision, as_stream(stream));
+}
+
+int nbx_flowmap_aux_grid_2d(const nbx_flow* fl, double t0, double T, const double* x, int nx,
+                            const double* y, int ny, double h, int eig_main, int compute_edge,
+                            const double* params, int np, double rtol, double atol,
+                            const uint8_t* mask, int64_t max_steps, int precision, double* out,
+                            int32_t* status, int32_t* nsteps, nbx_stream stream) {
+    if (int rc = check_solve_args(fl, params, np, rtol, atol, max_steps, precision)) return rc;
+    NBX_REQUIRE(nx >= 0 && ny >= 0, "negative grid size");
+    const int naux = eig_main ? 5 : 4;
+    const int64_t P = static_cast<int64_t>(nx) * ny * naux;
+    if (P == 0) return 0;
+    NBX_REQUIRE(x != nullptr && y != nullptr && out != nullptr, "NULL array");
+    SolveArgs a{P, params[0] * t0, params[0] * (t0 + T), rtol, atol, mask, max_steps,
+                reinterpret_cast<double2*>(out), status, nsteps};
+    const AuxSource src{x, y, nx, ny, naux, h, eig_main != 0, compute_edge != 0};
+    return solve(fl, params, np, src, a, precision, as_stream(stream));
 }
 
 int nbx_flowmap_pts(const nbx_flow* fl, double t0, double T, const double* pts, int64_t npts,

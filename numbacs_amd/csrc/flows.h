@@ -201,10 +201,12 @@ __device__ __forceinline__ double group_sum(double v) {
 //          forms, and the vector L1's tag rate (one line per clock) is what
 //          bounds these gathers.
 //
-// Out-of-box queries (parity unpinned: the reference's dependency is not
-// available): extrap 0/2 clamp the query into the box per axis (value of the
-// nearest box point), extrap 1 adds the gradient at that point times the
-// outward offset.
+// Out-of-box queries: extrap 0 ("constant") is zero outside the grid box,
+// the rule of the reference dependency's eval_linear (pinned by the
+// reference's composition fixture) carried over to its spline evaluation
+// (inferred: no reference test leaves the grid); extrap 2 ("nearest") clamps
+// the query into the box per axis; extrap 1 ("linear") adds the gradient at
+// the clamped point times the outward offset.
 template <class R, int G = 1>
 struct Spline3 {
     using P = typename CoefPair<R>::type;
@@ -386,8 +388,11 @@ struct Spline3 {
         const double cx = fmin(fmax(qx, lo[1]), hi[1]);
         const double cy = fmin(fmax(qy, lo[2]), hi[2]);
         const bool inside = (ct == qt) & (cx == qx) & (cy == qy);
-        if (inside || extrap != 1) {
+        if (inside || extrap == 2) {
             eval_in(ct, cx, cy, su, sv);
+        } else if (extrap == 0) {
+            su = 0;
+            sv = 0;
         } else {
             R g[3][2];
             eval_grad(ct, cx, cy, su, sv, g);

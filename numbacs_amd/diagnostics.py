@@ -1,6 +1,7 @@
 """FTLE and Cauchy-Green diagnostics (mirrors numbacs.diagnostics).
 
-Reference: ftle_grid_2D (diagnostics.py:21-65), C_eig_2D
+Reference: ftle_grid_2D (diagnostics.py:21-65), C_tensor_2D
+(diagnostics.py:67-110), C_eig_aux_2D (diagnostics.py:113-197), C_eig_2D
 (diagnostics.py:200-244) and ftle_from_eig (diagnostics.py:247-269).
 """
 import numpy as np
@@ -79,6 +80,52 @@ def C_eig_2D(flowmap, dx, dy, mask=None):
     _lib.check(_lib.load().nbx_C_eig_2d(_lib.ptr(fm), nx, ny, float(dx), float(dy),
                                         _lib.ptr(maskd), _lib.ptr(ev), _lib.ptr(V),
                                         _lib.stream_ptr()))
+    return _lib.to_host(ev), _lib.to_host(V)
+
+
+def _fm_aux_dev(flowmap_aux):
+    t = _lib.torch()
+    fa = _lib.to_dev(flowmap_aux if isinstance(flowmap_aux, t.Tensor) else
+                     np.asarray(flowmap_aux))
+    if fa.ndim != 4 or fa.shape[3] != 2 or fa.shape[2] not in (4, 5):
+        raise ValueError("flowmap_aux must have shape (nx, ny, n_aux, 2) with n_aux 4 or 5")
+    return fa
+
+
+def C_tensor_2D(flowmap_aux, dx, dy, h=1e-5, mask=None):
+    """(C11, C12, C22) of the Cauchy-Green tensor from the aux-grid flow map,
+    (nx, ny, 3); zero within two points of the border and at masked points
+    (dx, dy are unused, as in the reference)."""
+    fa = _fm_aux_dev(flowmap_aux)
+    nx, ny, naux = int(fa.shape[0]), int(fa.shape[1]), int(fa.shape[2])
+    if nx * ny == 0:
+        return np.zeros((nx, ny, 3))
+    C = _lib.empty((nx, ny, 3))
+    maskd = _mask_dev(mask, nx, ny)
+    _lib.check(_lib.load().nbx_C_tensor_aux_2d(_lib.ptr(fa), nx, ny, naux, float(h),
+                                               _lib.ptr(maskd), _lib.ptr(C), _lib.stream_ptr()))
+    return _lib.to_host(C)
+
+
+def C_eig_aux_2D(flowmap_aux, dx, dy, h=1e-5, eig_main=True, mask=None):
+    """Eigenvalues (nx, ny, 2) and eigenvectors (nx, ny, 2, 2) from the
+    aux-grid flow map: with eig_main the eigenvectors come from the aux-grid
+    tensor and the eigenvalues from the main-grid tensor of the centre
+    particles (zero within two points of the border); without it both come
+    from the aux-grid tensor (zero on the border).  Eigenvector signs may
+    differ from LAPACK's, as for C_eig_2D."""
+    fa = _fm_aux_dev(flowmap_aux)
+    nx, ny, naux = int(fa.shape[0]), int(fa.shape[1]), int(fa.shape[2])
+    if nx * ny == 0:
+        return np.zeros((nx, ny, 2)), np.zeros((nx, ny, 2, 2))
+    if eig_main and naux != 5:
+        raise ValueError("eig_main needs the centre particle: flowmap_aux with n_aux = 5")
+    ev = _lib.empty((nx, ny, 2))
+    V = _lib.empty((nx, ny, 2, 2))
+    maskd = _mask_dev(mask, nx, ny)
+    _lib.check(_lib.load().nbx_C_eig_aux_2d(_lib.ptr(fa), nx, ny, naux, float(dx), float(dy),
+                                            float(h), int(bool(eig_main)), _lib.ptr(maskd),
+                                            _lib.ptr(ev), _lib.ptr(V), _lib.stream_ptr()))
     return _lib.to_host(ev), _lib.to_host(V)
 
 

@@ -137,8 +137,8 @@ def get_flow_2D(grid_vel, C_eval_u, C_eval_v, spherical=0, extrap_mode="constant
 
     spherical: 0 planar; 1 lon in [-180,180) / 2 lon in [0,360) in degrees
     with velocities in km/h converted by radius r.  extrap_mode: 'constant'
-    (clamp the query into the grid box), 'linear' (first-order continuation
-    from the box) or 'nearest' (as 'constant')."""
+    (zero velocity outside the grid box), 'linear' (first-order continuation
+    from the box) or 'nearest' (clamp the query into the box)."""
     if spherical not in (0, 1, 2):
         raise ValueError("spherical must be 0, 1 or 2")
     if extrap_mode not in _lib.EXTRAP:

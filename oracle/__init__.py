@@ -65,6 +65,15 @@ def lib():
         L.orc_ridge_predicate.restype = _l
         L.orc_label8.argtypes = [_p, _i, _i, _p]
         L.orc_label8.restype = ctypes.c_int32
+        L.orc_flowmap_aux_grid_2d.argtypes = [_i, _p, _p, _p, _p, _i, _i, _d, _d, _d, _p, _i,
+                                              _p, _i, _d, _i, _i, _d, _d, _p, _l, _p, _i]
+        L.orc_flowmap_aux_grid_2d.restype = None
+        L.orc_C_tensor_aux_2d.argtypes = [_p, _i, _i, _i, _d, _p, _p]
+        L.orc_C_tensor_aux_2d.restype = None
+        L.orc_C_eig_aux_2d.argtypes = [_p, _i, _i, _i, _d, _d, _d, _i, _p, _p, _p]
+        L.orc_C_eig_aux_2d.restype = None
+        L.orc_flowmap_composition.argtypes = [_p, _i, _i, _i, _p, _p]
+        L.orc_flowmap_composition.restype = None
         L.orc_num_threads.argtypes = []
         L.orc_num_threads.restype = _i
         _lib = L
@@ -133,6 +142,56 @@ def flowmap_pts(kind, params, t0, T, pts, rtol=1e-6, atol=1e-8, spline=None, nma
                           float(T), _ptr(pts), n, float(rtol), float(atol), int(nmax), _ptr(out),
                           _ptr(status), int(nthreads))
     return out, status
+
+
+def flowmap_aux_grid_2d(kind, params, t0, T, x, y, h=1e-5, eig_main=True, compute_edge=True,
+                        rtol=1e-6, atol=1e-8, mask=None, spline=None, nmax=100000, nthreads=0):
+    """flowmap_aux_grid_2D restated (integration.py:249-464): (nx, ny, n_aux, 2)."""
+    x = _f64(x)
+    y = _f64(y)
+    nx, ny = len(x), len(y)
+    naux = 5 if eig_main else 4
+    out = np.zeros((nx, ny, naux, 2))
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    k, p, g, cu, cv, sph, ex, r = _flow_args(kind, params, spline)
+    lib().orc_flowmap_aux_grid_2d(k, _ptr(p), _ptr(g), _ptr(cu), _ptr(cv), sph, ex, r,
+                                  float(t0), float(T), _ptr(x), nx, _ptr(y), ny, float(h),
+                                  int(bool(eig_main)), int(bool(compute_edge)), float(rtol),
+                                  float(atol), _ptr(m), int(nmax), _ptr(out), int(nthreads))
+    return out
+
+
+def C_tensor_aux_2d(fm_aux, h=1e-5, mask=None):
+    """C_tensor_2D restated (diagnostics.py:67-110): (nx, ny, 3)."""
+    F = _f64(fm_aux)
+    nx, ny, naux = F.shape[:3]
+    C = np.zeros((nx, ny, 3))
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    lib().orc_C_tensor_aux_2d(_ptr(F), nx, ny, naux, float(h), _ptr(m), _ptr(C))
+    return C
+
+
+def C_eig_aux_2d(fm_aux, dx, dy, h=1e-5, eig_main=True, mask=None):
+    """C_eig_aux_2D restated (diagnostics.py:113-197)."""
+    F = _f64(fm_aux)
+    nx, ny, naux = F.shape[:3]
+    ev = np.zeros((nx, ny, 2))
+    V = np.zeros((nx, ny, 2, 2))
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    lib().orc_C_eig_aux_2d(_ptr(F), nx, ny, naux, float(dx), float(dy), float(h),
+                           int(bool(eig_main)), _ptr(m), _ptr(ev), _ptr(V))
+    return ev, V
+
+
+def flowmap_composition(flowmaps, grid, nT):
+    """flowmap_composition restated (integration.py:609-644); grid =
+    ((x0, x1, nx), (y0, y1, ny))."""
+    F = _f64(flowmaps)
+    nx, ny = int(grid[0][2]), int(grid[1][2])
+    g4 = np.array([grid[0][0], grid[0][1], grid[1][0], grid[1][1]], dtype=np.float64)
+    out = np.zeros((nx, ny, 2))
+    lib().orc_flowmap_composition(_ptr(F), int(nT), nx, ny, _ptr(g4), _ptr(out))
+    return out
 
 
 def rhs_eval(kind, params, t, pts, spline=None):

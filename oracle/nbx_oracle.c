@@ -31,8 +31,12 @@
  *   ftle / C_eig         : ftle.npy, Cevals.npy, Cevecs.npy fixtures;
  *   ridge predicate      : ridge_pts.npy fixture;
  *   percentile           : numpy.percentile; labelling: scipy.ndimage.label.
- *   Out-of-grid spline extrapolation: parity unpinned (no reference test
- *   leaves the grid; the chosen semantics are documented below).
+ *   Out-of-grid spline extrapolation: "constant" is zero outside the grid
+ *   box, the rule the same interpolation package's eval_linear follows
+ *   (pinned by the reference's composition fixture fm_ci.npy); for the
+ *   spline itself this is inferred, not pinned (no reference test leaves the
+ *   grid).  "nearest" clamps the query into the box, "linear" continues
+ *   with the gradient at the clamped point.
  */
 #include <math.h>
 #include <stdint.h>
@@ -260,6 +264,7 @@ double orc_spline3_eval(const orc_spline* g, const double* C, double t, double x
         out |= (qc[d] != q[d]);
     }
     if (!out) return spline3_eval_in(g, C, q, NULL);
+    if (g->extrap == 0) return 0.0;
     if (g->extrap == 1) {
         double gr[3];
         double v = spline3_eval_in(g, C, qc, gr);
@@ -545,6 +550,171 @@ void orc_flowmap_pts(int kind, const double* params, const double* grid9, const 
         out[2 * k] = yy[0];
         out[2 * k + 1] = yy[1];
         if (status) status[k] = idid;
+    }
+}
+
+/* flowmap_aux_grid_2D (integration.py:249-464): n_aux = 5 particles per
+ * grid point (offsets (h,0), (-h,0), (0,h), (0,-h), (0,0)) when eig_main,
+ * else the first 4.  compute_edge: edge points integrate only the centre
+ * particle (eig_main) or all 4 (not eig_main); without compute_edge only
+ * interior points are integrated.  Skipped and masked particles stay 0.
+ * out: (nx, ny, n_aux, 2).                                                 */
+void orc_flowmap_aux_grid_2d(int kind, const double* params, const double* grid9,
+                             const double* cu, const double* cv, int spherical, int extrap,
+                             double r, double t0, double T, const double* x, int nx,
+                             const double* y, int ny, double h, int eig_main, int compute_edge,
+                             double rtol, double atol, const uint8_t* mask, long nmax,
+                             double* out, int nthreads)
+{
+    orc_flow fl;
+    flow_init(&fl, kind, params, grid9, cu, cv, spherical, extrap, r);
+    orc_rhs_fn f;
+    const void* ctx;
+    flow_select(&fl, &f, &ctx);
+    const double s0 = params[0] * t0, s1 = params[0] * (t0 + T);
+    const int naux = eig_main ? 5 : 4;
+    const double off[5][2] = {{h, 0.0}, {-h, 0.0}, {0.0, h}, {0.0, -h}, {0.0, 0.0}};
+    memset(out, 0, sizeof(double) * (size_t)nx * ny * naux * 2);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int i = 0; i < nx; ++i) {
+        for (int j = 0; j < ny; ++j) {
+            long c = (long)i * ny + j;
+            if (mask && mask[c]) continue;
+            int edge = (i == 0 || i == nx - 1 || j == 0 || j == ny - 1);
+            if (edge && !compute_edge) continue;
+            for (int k = 0; k < naux; ++k) {
+                if (edge && eig_main && k != 4) continue;
+                double yy[2] = {x[i] + off[k][0], y[j] + off[k][1]};
+                orc_dop853_2d(f, ctx, s0, s1, yy, rtol, atol, nmax, NULL, NULL);
+                out[2 * (c * naux + k)] = yy[0];
+                out[2 * (c * naux + k) + 1] = yy[1];
+            }
+        }
+    }
+}
+
+static void eig2_sym(double a, double b, double d, double ev[2], double V[2][2]);
+
+/* gradF_aux_stencil_2D (utils.py:50-84) at cell c of F_aux (n_aux pairs) */
+static void grad_aux(const double* F, long c, int naux, double h, double g[4])
+{
+    const double* p = F + 2 * c * naux;
+    g[0] = (p[0] - p[2]) / (2 * h);  /* dFx/dx */
+    g[1] = (p[4] - p[6]) / (2 * h);  /* dFx/dy */
+    g[2] = (p[1] - p[3]) / (2 * h);  /* dFy/dx */
+    g[3] = (p[5] - p[7]) / (2 * h);  /* dFy/dy */
+}
+
+/* C_tensor_2D (diagnostics.py:67-110): (C11, C12, C22) on i, j in [2, n-3] */
+void orc_C_tensor_aux_2d(const double* F, int nx, int ny, int naux, double h,
+                         const uint8_t* mask, double* C)
+{
+    memset(C, 0, sizeof(double) * (size_t)nx * ny * 3);
+    for (int i = 2; i < nx - 2; ++i) {
+        for (int j = 2; j < ny - 2; ++j) {
+            long c = (long)i * ny + j;
+            if (mask && mask[c]) continue;
+            double g[4];
+            grad_aux(F, c, naux, h, g);
+            C[3 * c] = g[0] * g[0] + g[2] * g[2];
+            C[3 * c + 1] = g[0] * g[1] + g[2] * g[3];
+            C[3 * c + 2] = g[1] * g[1] + g[3] * g[3];
+        }
+    }
+}
+
+/* C_eig_aux_2D (diagnostics.py:113-197).  eig_main: on i, j in [2, n-3]
+ * the eigenvectors of the aux-grid tensor and the eigenvalues of the
+ * main-grid tensor (gradF_main_stencil_2D, utils.py:88-124, central
+ * differences of the centre particles); otherwise on i, j in [1, n-2] both
+ * from the aux-grid tensor.                                                */
+void orc_C_eig_aux_2d(const double* F, int nx, int ny, int naux, double dx, double dy, double h,
+                      int eig_main, const uint8_t* mask, double* evals, double* evecs)
+{
+    memset(evals, 0, sizeof(double) * (size_t)nx * ny * 2);
+    memset(evecs, 0, sizeof(double) * (size_t)nx * ny * 4);
+    const int lo = eig_main ? 2 : 1;
+    for (int i = lo; i < nx - lo; ++i) {
+        for (int j = lo; j < ny - lo; ++j) {
+            long c = (long)i * ny + j;
+            if (mask && mask[c]) continue;
+            double g[4];
+            grad_aux(F, c, naux, h, g);
+            double ev[2], V[2][2];
+            eig2_sym(g[0] * g[0] + g[2] * g[2], g[0] * g[1] + g[2] * g[3],
+                     g[1] * g[1] + g[3] * g[3], ev, V);
+            if (eig_main) {
+                /* centre particle (index n_aux - 1) of the four neighbours */
+                const int m = naux - 1;
+                const double* xp = F + 2 * ((c + ny) * naux + m);
+                const double* xm = F + 2 * ((c - ny) * naux + m);
+                const double* yp = F + 2 * ((c + 1) * naux + m);
+                const double* ym = F + 2 * ((c - 1) * naux + m);
+                double dxdx = (xp[0] - xm[0]) / (2 * dx);
+                double dxdy = (yp[0] - ym[0]) / (2 * dy);
+                double dydx = (xp[1] - xm[1]) / (2 * dx);
+                double dydy = (yp[1] - ym[1]) / (2 * dy);
+                double ev2[2], V2[2][2];
+                eig2_sym(dxdx * dxdx + dydx * dydx, dxdx * dxdy + dydx * dydy,
+                         dxdy * dxdy + dydy * dydy, ev2, V2);
+                ev[0] = ev2[0];
+                ev[1] = ev2[1];
+            }
+            evals[2 * c] = ev[0];
+            evals[2 * c + 1] = ev[1];
+            evecs[4 * c + 0] = V[0][0];
+            evecs[4 * c + 1] = V[0][1];
+            evecs[4 * c + 2] = V[1][0];
+            evecs[4 * c + 3] = V[1][1];
+        }
+    }
+}
+
+/* eval_linear on a uniform 2-D grid with CONSTANT extrapolation: 0 outside
+ * the closed box [x0, x1] x [y0, y1] (the rule pinned by the reference's
+ * composition fixture fm_ci.npy), bilinear inside with the cell index
+ * clamped to [0, n-2].  vals: (nx, ny) at stride vstride doubles.          */
+static double bilinear0(const double* vals, long vstride, int nx, int ny, const double* g4,
+                        double qx, double qy)
+{
+    if (qx < g4[0] || qx > g4[1] || qy < g4[2] || qy > g4[3]) return 0.0;
+    double ux = (qx - g4[0]) / ((g4[1] - g4[0]) / (nx - 1));
+    double uy = (qy - g4[2]) / ((g4[3] - g4[2]) / (ny - 1));
+    int i = (int)floor(ux), j = (int)floor(uy);
+    if (i > nx - 2) i = nx - 2;
+    if (j > ny - 2) j = ny - 2;
+    if (i < 0) i = 0;
+    if (j < 0) j = 0;
+    double lx = ux - i, ly = uy - j;
+    double v00 = vals[vstride * ((long)i * ny + j)], v01 = vals[vstride * ((long)i * ny + j + 1)];
+    double v10 = vals[vstride * ((long)(i + 1) * ny + j)];
+    double v11 = vals[vstride * ((long)(i + 1) * ny + j + 1)];
+    return (1 - lx) * ((1 - ly) * v00 + ly * v01) + lx * ((1 - ly) * v10 + ly * v11);
+}
+
+/* flowmap_composition (integration.py:609-644): the positions after the
+ * first interval pass through the interpolated flow maps 1 .. nT-2, then
+ * the last one (flowmaps[nT-1]; for nT = 1 that is flowmaps[0] again).
+ * flowmaps: (nT, nx, ny, 2); g4 = (x0, x1, y0, y1); out: (nx, ny, 2).      */
+void orc_flowmap_composition(const double* flowmaps, int nT, int nx, int ny, const double* g4,
+                             double* out)
+{
+    const long n = (long)nx * ny;
+    for (long p = 0; p < n; ++p) {
+        double px = flowmaps[2 * p], py = flowmaps[2 * p + 1];
+        for (int k = 1; k < nT - 1; ++k) {
+            const double* F = flowmaps + 2 * n * k;
+            double fx = bilinear0(F, 2, nx, ny, g4, px, py);
+            double fy = bilinear0(F + 1, 2, nx, ny, g4, px, py);
+            px = fx;
+            py = fy;
+        }
+        const double* L = flowmaps + 2 * n * (nT - 1);
+        out[2 * p] = bilinear0(L, 2, nx, ny, g4, px, py);
+        out[2 * p + 1] = bilinear0(L + 1, 2, nx, ny, g4, px, py);
     }
 }
 

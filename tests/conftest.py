@@ -55,6 +55,27 @@ def ridge_pts_data():
 
 
 @pytest.fixture
+def fm_aux_data():
+    return np.load(golden("fm_aux.npy"))
+
+
+@pytest.fixture
+def C_aux_data():
+    return np.load(golden("C.npy"))
+
+
+@pytest.fixture
+def C_eig_aux_data():
+    return np.load(golden("Cevals_aux.npy")), np.load(golden("Cevecs_aux.npy"))
+
+
+@pytest.fixture
+def composition_data():
+    """fm_ci, fms_ci, fm_cs, fms_cs (tests/test_integration.py:92-114)."""
+    return {k: np.load(golden(k + ".npy")) for k in ("fm_ci", "fms_ci", "fm_cs", "fms_cs")}
+
+
+@pytest.fixture
 def literals():
     return np.load(golden("reference_literals.npz"))
 

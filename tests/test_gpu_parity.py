@@ -99,10 +99,11 @@ def test_spline_velocity_literals(nb, literals):
     assert np.allclose(vel.T, literals["spline3_vel"])
 
 
-@pytest.mark.parametrize("extrap", ["constant", "linear"])
+@pytest.mark.parametrize("extrap", ["constant", "linear", "nearest"])
 def test_spline_velocity_out_of_grid_vs_oracle(nb, extrap):
-    """Extrapolated values follow the oracle's stated semantics (the
-    reference's out-of-grid behaviour is unpinned)."""
+    """Extrapolated values follow the oracle's semantics (constant: zero
+    outside the box, as the dependency's eval_linear; linear and nearest are
+    unpinned by the reference)."""
     from numbacs_amd.flows import flow_velocity, get_flow_2D, get_interp_arrays_2D
 
     rng = np.random.default_rng(5)

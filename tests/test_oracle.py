@@ -149,12 +149,19 @@ def test_spline_reproduces_cubics():
     grid = ((0.0, 3.0, 7), (-1.0, 1.0, 9), (0.0, 2.0, 8))
     exact = 0.3 * q[:, 0] - 1.7 * q[:, 1] + 2.5 * q[:, 2] + 0.25
     assert np.abs(oracle.spline3_eval(grid, C, q) - exact).max() < 1e-12
-    # constant extrapolation clamps; linear extrapolation continues the plane
+    # out of the box: constant is zero, nearest clamps, linear continues the
+    # plane
     qo = q + np.array([4.0, 0.0, 0.0])
+    assert np.array_equal(oracle.spline3_eval(grid, C, qo, "constant"), np.zeros(len(q)))
     clamp = 0.3 * 3.0 - 1.7 * q[:, 1] + 2.5 * q[:, 2] + 0.25
-    assert np.abs(oracle.spline3_eval(grid, C, qo, "constant") - clamp).max() < 1e-12
+    assert np.abs(oracle.spline3_eval(grid, C, qo, "nearest") - clamp).max() < 1e-12
     lin = 0.3 * qo[:, 0] - 1.7 * q[:, 1] + 2.5 * q[:, 2] + 0.25
     assert np.abs(oracle.spline3_eval(grid, C, qo, "linear") - lin).max() < 1e-11
+    # the box is closed: its faces are inside for every mode
+    qb = q.copy()
+    qb[:, 0] = 3.0
+    for mode in ("constant", "nearest", "linear"):
+        assert np.abs(oracle.spline3_eval(grid, C, qb, mode) - clamp).max() < 1e-12
 
 
 def test_ftle_fixture(coords_dg, fm_data, ftle_data, mask_dg):
@@ -241,3 +248,126 @@ def test_label8_vs_scipy(seed):
     lab, n = oracle.label8(b)
     ref, nref = ndimage.label(b, structure=ndimage.generate_binary_structure(2, 2))
     assert n == nref and np.array_equal(lab, ref)
+
+
+# ------------------------------------------------- aux grid, composition --
+def test_flowmap_aux_fixture(coords_dg, fm_aux_data, mask_dg):
+    """fm_aux.npy (tests/test_integration.py:66-75): every value equal after
+    the float32 cast; masked points 0."""
+    x, y = coords_dg
+    fa = oracle.flowmap_aux_grid_2d(DG, dg_params(), 0.0, 8.0, x, y)
+    assert np.array_equal(fa.astype(np.float32), fm_aux_data)
+    fam = oracle.flowmap_aux_grid_2d(DG, dg_params(), 0.0, 8.0, x, y, mask=mask_dg)
+    assert np.allclose(fam.astype(np.float32), apply_mask(fm_aux_data, mask_dg))
+
+
+def test_flowmap_aux_edge_rules(coords_dg):
+    """Edge points carry only the centre particle (eig_main) or all four
+    (not eig_main); compute_edge=False leaves the border at 0.  Every
+    integrated particle equals the point-list flow map of its start."""
+    x, y = coords_dg
+    h = 1e-3
+    for eig_main in (True, False):
+        for edge in (True, False):
+            fa = oracle.flowmap_aux_grid_2d(DG, dg_params(), 0.0, 2.0, x, y, h=h,
+                                            eig_main=eig_main, compute_edge=edge)
+            naux = 5 if eig_main else 4
+            assert fa.shape == (len(x), len(y), naux, 2)
+            off = np.array([[h, 0], [-h, 0], [0, h], [0, -h], [0, 0]])[:naux]
+            X, Y = np.meshgrid(x, y, indexing="ij")
+            starts = np.stack([X, Y], -1)[:, :, None, :] + off[None, None]
+            want = np.ones((len(x), len(y), naux), bool)
+            border = np.zeros((len(x), len(y)), bool)
+            border[[0, -1], :] = True
+            border[:, [0, -1]] = True
+            if not edge:
+                want[border] = False
+            elif eig_main:
+                want[border, :4] = False
+            ref, _ = oracle.flowmap_pts(DG, dg_params(), 0.0, 2.0, starts[want])
+            assert np.array_equal(fa[want], ref)
+            assert not fa[~want].any()
+
+
+def test_C_tensor_aux_fixture(coords_dg, fm_aux_data, C_aux_data, mask_dg):
+    """C.npy (tests/test_diagnostics.py:99-110)."""
+    C = oracle.C_tensor_aux_2d(fm_aux_data)
+    assert np.allclose(C.astype(np.float32), C_aux_data)
+    Cm = oracle.C_tensor_aux_2d(fm_aux_data, mask=mask_dg)
+    assert np.allclose(Cm.astype(np.float32), apply_mask(C_aux_data, mask_dg))
+
+
+def test_C_eig_aux_fixture(coords_dg, fm_aux_data, C_eig_aux_data, mask_dg):
+    """Cevals_aux.npy / Cevecs_aux.npy compared as V diag(l) V^T
+    (tests/test_diagnostics.py:112-131)."""
+    x, y = coords_dg
+    ev, V = oracle.C_eig_aux_2d(fm_aux_data, x[1], y[1])
+    want = reconstruct(*C_eig_aux_data)
+    got = reconstruct(ev.astype(np.float32), V.astype(np.float32))
+    assert np.allclose(got, want)
+    assert np.allclose(ev.astype(np.float32), C_eig_aux_data[0])
+    evm, Vm = oracle.C_eig_aux_2d(fm_aux_data, x[1], y[1], mask=mask_dg)
+    assert np.allclose(reconstruct(evm.astype(np.float32), Vm.astype(np.float32)),
+                       apply_mask(want, mask_dg))
+
+
+def test_C_eig_aux_without_main_matches_numpy(fm_aux_data, coords_dg):
+    """eig_main=False: both from the aux tensor, on [1, n-2]; eigenvalues
+    against numpy.linalg.eigvalsh of the tensor C_tensor_2D restates."""
+    x, y = coords_dg
+    F = fm_aux_data.astype(np.float64)[:, :, :4]
+    ev, V = oracle.C_eig_aux_2d(F, x[1], y[1], eig_main=False)
+    h = 1e-5
+    g = np.stack([(F[..., 0, 0] - F[..., 1, 0]) / (2 * h), (F[..., 2, 0] - F[..., 3, 0]) / (2 * h),
+                  (F[..., 0, 1] - F[..., 1, 1]) / (2 * h), (F[..., 2, 1] - F[..., 3, 1]) / (2 * h)],
+                 -1)
+    C = np.empty(F.shape[:2] + (2, 2))
+    C[..., 0, 0] = g[..., 0] ** 2 + g[..., 2] ** 2
+    C[..., 0, 1] = C[..., 1, 0] = g[..., 0] * g[..., 1] + g[..., 2] * g[..., 3]
+    C[..., 1, 1] = g[..., 1] ** 2 + g[..., 3] ** 2
+    inner = (slice(1, -1), slice(1, -1))
+    assert np.allclose(ev[inner], np.linalg.eigvalsh(C[inner]), rtol=1e-10, atol=1e-12)
+    assert not ev[0].any() and not ev[:, 0].any()
+
+
+def test_composition_fixtures(coords_dg, composition_data):
+    """flowmap_composition_initial / _step (tests/test_integration.py:92-114):
+    eight 1-unit flow maps of the double gyre from t0 = 0 composed over
+    T = 8; the step shifts the window by one.  The CONSTANT extrapolation of
+    the interpolation is zero outside the grid box: five trajectories end
+    1.8e-15 beyond x = 2, and the fixture holds 0 there."""
+    x, y = coords_dg
+    grid = ((0.0, 2.0, len(x)), (0.0, 1.0, len(y)))
+    fms = np.stack([oracle.flowmap_grid_2d(DG, dg_params(), float(k), 1.0, x, y)[0]
+                    for k in range(8)])
+    assert np.allclose(fms.astype(np.float32), composition_data["fms_ci"])
+    c0 = oracle.flowmap_composition(fms, grid, 8)
+    assert np.allclose(c0.astype(np.float32), composition_data["fm_ci"])
+    assert (fms[..., 0] > 2.0).sum() == 5 and not c0[-1, 1:].any()
+    # the step starts from the float32 fixture, as the reference test does
+    fcs = composition_data["fms_ci"].astype(np.float64)
+    fcs = np.concatenate([fcs[1:], oracle.flowmap_grid_2d(DG, dg_params(), 8.0, 1.0, x, y)[0][None]])
+    assert np.allclose(fcs.astype(np.float32), composition_data["fms_cs"])
+    assert np.allclose(oracle.flowmap_composition(fcs, grid, 8).astype(np.float32),
+                       composition_data["fm_cs"])
+
+
+def test_composition_of_linear_maps_is_exact():
+    """Affine maps that keep the box are reproduced exactly by bilinear
+    interpolation, so composing them equals their product; nT = 1 applies
+    the single map twice (the reference's loop bounds)."""
+    nx, ny = 9, 7
+    x, y = np.linspace(0, 2, nx), np.linspace(0, 1, ny)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    grid = ((0.0, 2.0, nx), (0.0, 1.0, ny))
+    maps = [(0.9, 0.05, 0.02, 0.01), (0.8, 0.1, 0.1, 0.05), (0.95, 0.0, 0.03, 0.02)]
+    F = np.stack([np.stack([a * X + c, b * Y + d], -1) for a, c, b, d in
+                  [(m[0], m[2], m[0], m[3]) for m in maps]])
+    px, py = X, Y
+    for a, c, b, d in [(m[0], m[2], m[0], m[3]) for m in maps]:
+        px, py = a * px + c, b * py + d
+    got = oracle.flowmap_composition(F, grid, 3)
+    assert np.abs(got - np.stack([px, py], -1)).max() < 1e-13
+    one = oracle.flowmap_composition(F[:1], grid, 1)
+    a, c, b, d = maps[0][0], maps[0][2], maps[0][0], maps[0][3]
+    assert np.abs(one - np.stack([a * (a * X + c) + c, b * (b * Y + d) + d], -1)).max() < 1e-13
