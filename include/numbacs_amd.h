@@ -33,7 +33,12 @@ extern "C" {
 typedef struct nbx_flow nbx_flow;
 typedef void* nbx_stream; /* a hipStream_t */
 
-enum nbx_flow_kind { NBX_FLOW_DOUBLE_GYRE = 0, NBX_FLOW_BICKLEY_JET = 1, NBX_FLOW_SPLINE3 = 2 };
+enum nbx_flow_kind {
+    NBX_FLOW_DOUBLE_GYRE = 0,
+    NBX_FLOW_BICKLEY_JET = 1,
+    NBX_FLOW_SPLINE3 = 2,
+    NBX_FLOW_LINEAR3 = 3
+};
 enum nbx_extrap { NBX_EXTRAP_CONSTANT = 0, NBX_EXTRAP_LINEAR = 1, NBX_EXTRAP_NEAREST = 2 };
 enum nbx_precision { NBX_FP64 = 0, NBX_FP32_DISP = 1 };
 
@@ -64,6 +69,20 @@ int nbx_flow_create_analytic(int kind, nbx_flow** out);
 int nbx_flow_create_spline3(const double* grid9_host, const double* Cu, const double* Cv,
                             int spherical, int extrap, double r, nbx_flow** out,
                             nbx_stream stream);
+
+/* A gridded flow evaluated by trilinear interpolation of the samples
+ * (get_flow_linear_2D, flows.py:418-506): U/V = DEVICE (nt, nx, ny) float64
+ * velocity samples on the grid grid9_host; spherical and r as for
+ * nbx_flow_create_spline3; extrap: 0 zero outside the grid box (the
+ * reference dependency's CONSTANT), 1 continued boundary cell, 2 clamped. */
+int nbx_flow_create_linear3(const double* grid9_host, const double* U, const double* V,
+                            int spherical, int extrap, double r, nbx_flow** out,
+                            nbx_stream stream);
+
+/* Velocity ingest: (nt, ny, nx) 'xy' samples -> (nt, nx, ny) 'ij' times
+ * scale (the np.moveaxis(u, 1, 2) * 3.6 of plot_merra_ftle.py:37-41). */
+int nbx_ingest_xy(const double* U_xy, int nt, int ny, int nx, double scale, double* U_ij,
+                  nbx_stream stream);
 void nbx_flow_destroy(nbx_flow* flow);
 
 /* Natural cubic B-spline prefilter (get_interp_arrays_2D, flows.py:39-46):
@@ -82,7 +101,8 @@ int nbx_flow_eval(const nbx_flow* flow, const double* params_host, int nparams, 
  * skip, may be NULL) leaves 0 in out.  status / nsteps (nx*ny int32, may be
  * NULL) receive nbx_status and the attempted step count.  max_steps bounds
  * every solve (Hairer's NMAX).  precision selects FP64 or the
- * FP32-displacement form (FP32 state + spline, FP64 step control). */
+ * FP32-displacement form (FP32 stages, step displacements and spline;
+ * FP64 position, time and step control). */
 int nbx_flowmap_grid_2d(const nbx_flow* flow, double t0, double T, const double* x, int nx,
                         const double* y, int ny, const double* params_host, int nparams,
                         double rtol, double atol, const uint8_t* mask, int64_t max_steps,

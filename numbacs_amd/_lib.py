@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libnumbacs_amd.so")
 NBX_FLOW_DOUBLE_GYRE = 0
 NBX_FLOW_BICKLEY_JET = 1
 NBX_FLOW_SPLINE3 = 2
+NBX_FLOW_LINEAR3 = 3
 
 NBX_FP64 = 0
 NBX_FP32_DISP = 1
@@ -33,6 +34,8 @@ SIGNATURES = {
     "nbx_version": (_c_i, []),
     "nbx_flow_create_analytic": (_c_i, [_c_i, _c_p]),
     "nbx_flow_create_spline3": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_p, _c_p]),
+    "nbx_flow_create_linear3": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_p, _c_p]),
+    "nbx_ingest_xy": (_c_i, [_c_p, _c_i, _c_i, _c_i, _c_d, _c_p, _c_p]),
     "nbx_flow_destroy": (None, [_c_p]),
     "nbx_prefilter3": (_c_i, [_c_p, _c_i, _c_i, _c_i, _c_p, _c_p]),
     "nbx_flow_eval": (_c_i, [_c_p, _c_p, _c_i, _c_p, _c_p, _c_i64, _c_p, _c_p]),

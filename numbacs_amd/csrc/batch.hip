@@ -104,6 +104,8 @@ int launch_batch_flow(nbx_batch* b, hipStream_t s) {
         if (cfg.lanes == 2) return launch_batch_queue<Spline3<R, 2>, R, 2>(b, regroup<2>(sp), cfg, s);
         return launch_batch_queue<Spline3<R, 1>, R, 1>(b, sp, cfg, s);
     }
+    if (fl->kind == NBX_FLOW_LINEAR3)
+        return launch_batch_queue<Linear3<R>, R, 1>(b, make_linear<R>(fl, p[0]), cfg, s);
     if (fl->kind == NBX_FLOW_DOUBLE_GYRE) {
         const DoubleGyre<R> dg{p[0], p[1], p[2], p[3], p[4], p[5]};
         return launch_batch_queue<DoubleGyre<R>, R, 1>(b, dg, cfg, s);

@@ -14,13 +14,12 @@
 // Precision forms (R = stage precision):
 //   R = double: the state is the absolute position (anchor 0), every
 //               operation in the order of Hairer's code;
-//   R = float : the state is a float displacement from a double anchor (the
-//               initial position) and the stored stages and the velocity
-//               evaluation (spline gathers and sums) are float, while time,
+//   R = float : the stored stages (velocities) and the velocity evaluation
+//               (spline gathers and sums) are float; the position is a double
+//               anchor, moved to each accepted point, plus the step's
+//               displacement h * sum(b k) formed from the float stages; time,
 //               step size, the stage sums, the error norm and the step-size
-//               control stay double.  Storing displacement keeps the float
-//               rounding relative to the distance travelled, not to
-//               |position|.
+//               control stay double.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -288,11 +287,22 @@ __device__ __forceinline__ bool dop853_step(const Flow& f, Dop853State<R>& s, do
     double hnew = h / fac;
     if (err <= 1.0) {
         s.facold = fmax(err, 1.0e-4);
-        // new state (exact for double, where the anchor is 0), then the
-        // derivative at the stored point
-        s.dx = static_cast<R>(pnx - ax);
-        s.dy = static_cast<R>(pny - ay);
-        f(xph, ax + D(s.dx), ay + D(s.dy), s.k1x, s.k1y);
+        // new state, then the derivative at the stored point.  The float
+        // form moves its double anchor to the accepted position: a float
+        // displacement from the initial point loses increments below its
+        // resolution (~1e-7 of the distance travelled), and a particle whose
+        // steps shrink at a velocity discontinuity (the grid edge under zero
+        // extrapolation) would stop moving while time crawls on.
+        if constexpr (sizeof(R) == sizeof(double)) {
+            s.dx = pnx;
+            s.dy = pny;
+        } else {
+            s.ax = pnx;
+            s.ay = pny;
+            s.dx = R(0);
+            s.dy = R(0);
+        }
+        f(xph, s.px(), s.py(), s.k1x, s.k1y);
         s.x = xph;
         if (s.last) {
             s.idid = 1;

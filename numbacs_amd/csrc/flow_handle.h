@@ -10,8 +10,9 @@
 
 struct nbx_flow {
     int kind;
-    // spline flows
-    double2* coef = nullptr;  // (nt+2, nx+2, ny+2) interleaved (u, v), device
+    // gridded flows
+    double2* coef = nullptr;  // spline: (nt+2, nx+2, ny+2) interleaved (u, v) coefficients;
+                              // trilinear: (nt, nx, ny) interleaved samples; device
     float4* coef32 = nullptr; // FP32 copy, time-paired, for the FP32-displacement path
     int64_t ncoef = 0;
     double grid9[9] = {0};
@@ -52,6 +53,25 @@ Spline3<R, 1> make_spline(const nbx_flow* fl, double p0) {
     const int nx = static_cast<int>(fl->grid9[5]), ny = static_cast<int>(fl->grid9[8]);
     s.sx = ny + 2;
     s.st = (nx + 2) * (ny + 2);
+    s.spherical = fl->spherical;
+    s.extrap = fl->extrap;
+    s.r = fl->r;
+    s.p0 = p0;
+    return s;
+}
+
+template <class R>
+Linear3<R> make_linear(const nbx_flow* fl, double p0) {
+    Linear3<R> s;
+    s.V = fl->coef;
+    for (int d = 0; d < 3; ++d) {
+        s.lo[d] = fl->grid9[3 * d];
+        s.hi[d] = fl->grid9[3 * d + 1];
+        s.n[d] = static_cast<int>(fl->grid9[3 * d + 2]);
+        s.step[d] = (s.hi[d] - s.lo[d]) / (s.n[d] - 1);
+    }
+    s.sx = s.n[2];
+    s.st = s.n[1] * s.n[2];
     s.spherical = fl->spherical;
     s.extrap = fl->extrap;
     s.r = fl->r;

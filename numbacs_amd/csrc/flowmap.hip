@@ -163,10 +163,27 @@ __global__ void k_interleave(const double* __restrict__ u, const double* __restr
     for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
          k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const double a = u[k], b = v[k];
-        const int64_t k1 = k + slab < n ? k + slab : k;
         out[k] = make_double2(a, b);
+        if (out32 == nullptr) continue;
+        const int64_t k1 = k + slab < n ? k + slab : k;
         out32[k] = make_float4(static_cast<float>(a), static_cast<float>(b),
                                static_cast<float>(u[k1]), static_cast<float>(v[k1]));
+    }
+}
+
+// 'xy' velocity samples (nt, ny, nx) to the 'ij' layout (nt, nx, ny), scaled
+// (the moveaxis + unit conversion of plot_merra_ftle.py:37-41: m/s -> km/h
+// is scale 3.6).  Writes are coalesced along y; the reads stride by nx.
+__global__ void k_ingest_xy(const double* __restrict__ in, int nt, int ny, int nx, double scale,
+                            double* __restrict__ out) {
+    const int64_t n = static_cast<int64_t>(nt) * ny * nx;
+    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t j = k % ny;
+        const int64_t ti = k / ny;
+        const int64_t i = ti % nx;
+        const int64_t t = ti / nx;
+        out[k] = in[(t * ny + j) * nx + i] * scale;
     }
 }
 
@@ -274,9 +291,10 @@ __global__ void k_flow_eval(Flow f, const double* __restrict__ t, const double2*
 // defaults: the best shapes measured on the gridded-velocity and
 // double-gyre workloads (spline flows: 4 lanes per particle, 128-thread
 // blocks); the autotuner overrides them per workload
-LaunchCfg g_cfg[3][2] = {{{1, 256, 0}, {1, 256, 0}},
+LaunchCfg g_cfg[4][2] = {{{1, 256, 0}, {1, 256, 0}},
                          {{1, 256, 0}, {1, 256, 0}},
-                         {{4, 128, 0}, {4, 128, 0}}};
+                         {{4, 128, 0}, {4, 128, 0}},
+                         {{1, 128, 0}, {1, 128, 0}}};
 std::mutex g_cfg_mutex;
 
 LaunchCfg launch_cfg(int kind, int precision) {
@@ -355,6 +373,8 @@ int dispatch_solve(const nbx_flow* fl, const double* params, int np, const Src& 
         if (cfg.lanes == 2) return launch_solve<Spline3<R, 2>, R, 2>(regroup<2>(sp), src, a, cfg, s);
         return launch_solve<Spline3<R, 1>, R, 1>(sp, src, a, cfg, s);
     }
+    if (fl->kind == NBX_FLOW_LINEAR3)
+        return launch_solve<Linear3<R>, R, 1>(make_linear<R>(fl, params[0]), src, a, cfg, s);
     if (fl->kind == NBX_FLOW_DOUBLE_GYRE) {
         NBX_REQUIRE(np >= 6, "double_gyre needs 6 parameters [dir, A, eps, alpha, omega, psi]");
         const DoubleGyre<R> dg{params[0], params[1], params[2], params[3], params[4], params[5]};
@@ -438,6 +458,58 @@ int nbx_flow_create_spline3(const double* grid9_host, const double* Cu, const do
     return 0;
 }
 
+int nbx_flow_create_linear3(const double* grid9_host, const double* U, const double* V,
+                            int spherical, int extrap, double r, nbx_flow** out,
+                            nbx_stream stream) {
+    NBX_REQUIRE(out != nullptr && grid9_host != nullptr && U != nullptr && V != nullptr,
+                "NULL argument");
+    NBX_REQUIRE(spherical >= 0 && spherical <= 2, "spherical must be 0, 1 or 2");
+    NBX_REQUIRE(extrap >= 0 && extrap <= 2, "unknown extrapolation mode");
+    int n[3];
+    for (int d = 0; d < 3; ++d) {
+        n[d] = static_cast<int>(grid9_host[3 * d + 2]);
+        NBX_REQUIRE(n[d] >= 2 && n[d] == grid9_host[3 * d + 2],
+                    "trilinear grids need at least 2 points per axis");
+        NBX_REQUIRE(grid9_host[3 * d + 1] > grid9_host[3 * d], "grid axes must be ascending");
+    }
+    const int64_t nval = static_cast<int64_t>(n[0]) * n[1] * n[2];
+    nbx_flow* fl = new nbx_flow();
+    fl->kind = NBX_FLOW_LINEAR3;
+    std::memcpy(fl->grid9, grid9_host, sizeof(fl->grid9));
+    fl->spherical = spherical;
+    fl->extrap = extrap;
+    fl->r = r;
+    fl->ncoef = nval;
+    if (hipMalloc(&fl->coef, sizeof(double2) * nval) != hipSuccess) {
+        nbx_flow_destroy(fl);
+        set_error("hipMalloc of velocity samples failed");
+        return -1;
+    }
+    hipStream_t s = as_stream(stream);
+    k_interleave<<<blocks_for(nval, 256) < 4096 ? blocks_for(nval, 256) : 4096, 256, 0, s>>>(
+        U, V, nval, 0, fl->coef, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        nbx_flow_destroy(fl);
+        set_error(std::string("interleave launch: ") + hipGetErrorString(e));
+        return -1;
+    }
+    *out = fl;
+    return 0;
+}
+
+int nbx_ingest_xy(const double* U_xy, int nt, int ny, int nx, double scale, double* U_ij,
+                  nbx_stream stream) {
+    NBX_REQUIRE(nt >= 0 && ny >= 0 && nx >= 0, "negative size");
+    const int64_t n = static_cast<int64_t>(nt) * ny * nx;
+    if (n == 0) return 0;
+    NBX_REQUIRE(U_xy != nullptr && U_ij != nullptr && U_xy != U_ij, "NULL or aliased array");
+    k_ingest_xy<<<blocks_for(n, 256) < 8192 ? blocks_for(n, 256) : 8192, 256, 0,
+                  as_stream(stream)>>>(U_xy, nt, ny, nx, scale, U_ij);
+    NBX_LAUNCHED("ingest_xy");
+    return 0;
+}
+
 void nbx_flow_destroy(nbx_flow* fl) {
     if (!fl) return;
     if (fl->coef) (void)hipFree(fl->coef);
@@ -477,6 +549,9 @@ int nbx_flow_eval(const nbx_flow* fl, const double* params, int np, const double
     if (fl->kind == NBX_FLOW_SPLINE3) {
         k_flow_eval<Spline3<double, 1>, double><<<nb, 256, 0, s>>>(
             make_spline<double>(fl, params[0]), t, P, n, O);
+    } else if (fl->kind == NBX_FLOW_LINEAR3) {
+        k_flow_eval<Linear3<double>, double><<<nb, 256, 0, s>>>(
+            make_linear<double>(fl, params[0]), t, P, n, O);
     } else if (fl->kind == NBX_FLOW_DOUBLE_GYRE) {
         NBX_REQUIRE(np >= 6, "double_gyre needs 6 parameters");
         const DoubleGyre<double> dg{params[0], params[1], params[2], params[3], params[4], params[5]};
@@ -548,7 +623,7 @@ int nbx_flowmap_pts(const nbx_flow* fl, double t0, double T, const double* pts, 
 
 int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int block_threads,
                           int persistent_blocks_per_cu) {
-    NBX_REQUIRE(kind >= 0 && kind <= 2, "unknown flow kind");
+    NBX_REQUIRE(kind >= 0 && kind <= 3, "unknown flow kind");
     NBX_REQUIRE(precision == NBX_FP64 || precision == NBX_FP32_DISP, "unknown precision");
     NBX_REQUIRE(lanes_per_particle == 1 || lanes_per_particle == 2 || lanes_per_particle == 4,
                 "lanes_per_particle must be 1, 2 or 4");
@@ -565,7 +640,7 @@ int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int b
 
 int nbx_get_launch_config(int kind, int precision, int* lanes_per_particle, int* block_threads,
                           int* persistent_blocks_per_cu) {
-    NBX_REQUIRE(kind >= 0 && kind <= 2, "unknown flow kind");
+    NBX_REQUIRE(kind >= 0 && kind <= 3, "unknown flow kind");
     NBX_REQUIRE(precision == NBX_FP64 || precision == NBX_FP32_DISP, "unknown precision");
     NBX_REQUIRE(lanes_per_particle != nullptr && block_threads != nullptr &&
                     persistent_blocks_per_cu != nullptr,

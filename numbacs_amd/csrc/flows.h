@@ -423,4 +423,77 @@ struct Spline3 {
     }
 };
 
+// Trilinear velocity on a uniform (t, x, y) grid (the reference's
+// get_flow_linear_2D over interpolation.eval_linear, flows.py:418-506).
+// V holds the (nt, nx, ny) samples as (u, v) pairs, y fastest.  Cell index
+// floor((q - lo)/step) clamped to [0, n-2], weights q/step - index, the
+// three linear stages in source order with contraction off.  Out-of-box
+// queries: extrap 0 ("constant") is zero, pinned for eval_linear by the
+// reference's composition fixture; 2 ("nearest") clamps the query into the
+// box; 1 ("linear") continues the boundary cell's trilinear function.  The
+// evaluation is FP64 in both precision forms (8 gathers of 16 B: not the
+// bottleneck), rounded to R at the end.
+template <class R>
+struct Linear3 {
+    const double2* __restrict__ V;
+    double lo[3], hi[3], step[3];
+    int n[3];
+    int sx, st;  // strides (in pairs) of the x and t axes
+    int spherical, extrap;
+    double r, p0;
+
+    __device__ __forceinline__ void eval(double qt, double qx, double qy, double& su,
+                                         double& sv) const {
+#pragma clang fp contract(off)
+        const double q[3] = {qt, qx, qy};
+        bool out = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out |= (q[d] < lo[d]) | (q[d] > hi[d]);
+        if (out && extrap == 0) {
+            su = 0.0;
+            sv = 0.0;
+            return;
+        }
+        int i[3];
+        double w[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double qq = extrap == 2 ? fmin(fmax(q[d], lo[d]), hi[d]) : q[d];
+            const double u = (qq - lo[d]) / step[d];
+            int k = static_cast<int>(floor(u));
+            k = k > n[d] - 2 ? n[d] - 2 : k;
+            k = k < 0 ? 0 : k;
+            i[d] = k;
+            w[d] = u - k;
+        }
+        const double2* c = V + (static_cast<long>(i[0]) * st + static_cast<long>(i[1]) * sx + i[2]);
+        const double2 a00 = c[0], a01 = c[1], a10 = c[sx], a11 = c[sx + 1];
+        const double2 b00 = c[st], b01 = c[st + 1], b10 = c[st + sx], b11 = c[st + sx + 1];
+        const double w1 = w[1], w2 = w[2], w0 = w[0];
+        const double bu0 = (1 - w1) * ((1 - w2) * a00.x + w2 * a01.x) + w1 * ((1 - w2) * a10.x + w2 * a11.x);
+        const double bv0 = (1 - w1) * ((1 - w2) * a00.y + w2 * a01.y) + w1 * ((1 - w2) * a10.y + w2 * a11.y);
+        const double bu1 = (1 - w1) * ((1 - w2) * b00.x + w2 * b01.x) + w1 * ((1 - w2) * b10.x + w2 * b11.x);
+        const double bv1 = (1 - w1) * ((1 - w2) * b00.y + w2 * b01.y) + w1 * ((1 - w2) * b10.y + w2 * b11.y);
+        su = (1 - w0) * bu0 + w0 * bu1;
+        sv = (1 - w0) * bv0 + w0 * bv1;
+    }
+
+    __device__ __forceinline__ void operator()(double t, double y0, double y1, R& f0,
+                                               R& f1) const {
+        const double tt = p0 * t;
+        double xx = y0;
+        if (spherical == 1) xx = py_mod(y0 - 180, 360) - 180;
+        else if (spherical != 0) xx = py_mod(y0, 360);
+        double su, sv;
+        eval(tt, xx, y1, su, sv);
+        if (spherical == 0) {
+            f0 = static_cast<R>(p0 * su);
+            f1 = static_cast<R>(p0 * sv);
+        } else {
+            f0 = static_cast<R>(((p0 * su) * 180) / (kPi * r * cos(y1 * kPi / 180)));
+            f1 = static_cast<R>(((p0 * sv) * 180) / (kPi * r));
+        }
+    }
+};
+
 }  // namespace nbx

@@ -1,7 +1,8 @@
 """Velocity fields for the flow-map path (mirrors numbacs.flows).
 
 Reference: get_predefined_flow (flows.py:1104-1295), get_interp_arrays_2D
-(flows.py:9-46) and get_flow_2D (flows.py:121-258).  The reference returns
+(flows.py:9-46), get_flow_2D (flows.py:121-258) and get_flow_linear_2D
+(flows.py:418-506).  The reference returns
 the address of a numba cfunc as ``funcptr``; here ``funcptr`` is a
 :class:`FlowHandle` -- an ``int`` (the address of an opaque ``nbx_flow``)
 that owns the device-side description of the flow.  A raw cfunc address from
@@ -156,6 +157,47 @@ def get_flow_2D(grid_vel, C_eval_u, C_eval_v, spherical=0, extrap_mode="constant
         grid9.ctypes.data, _lib.ptr(Cu), _lib.ptr(Cv), int(spherical), _lib.EXTRAP[extrap_mode],
         float(r), ctypes.byref(out), _lib.stream_ptr()))
     return FlowHandle(out.value, _lib.NBX_FLOW_SPLINE3, grid=grid_vel)
+
+
+def get_flow_linear_2D(grid_vel, U, V, spherical=0, extrap_mode="constant", r=6371.0):
+    """Trilinear flow over gridded velocity samples (flows.py:418-506).
+
+    U, V: (nt, nx, ny) samples on grid_vel ((t0,t1,nt),(x0,x1,nx),(y0,y1,ny)).
+    spherical and r as for get_flow_2D.  extrap_mode: 'constant' (zero
+    outside the grid box, the interpolation package's CONSTANT), 'linear'
+    (the boundary cell continued) or 'nearest' (clamp into the box)."""
+    if spherical not in (0, 1, 2):
+        raise ValueError("spherical must be 0, 1 or 2")
+    if extrap_mode not in _lib.EXTRAP:
+        raise ValueError(f"unknown extrap_mode {extrap_mode!r}")
+    grid9 = np.array([float(v) for ax in grid_vel for v in ax], dtype=np.float64)
+    if grid9.size != 9:
+        raise ValueError("grid_vel must be ((t0,t1,nt),(x0,x1,nx),(y0,y1,ny))")
+    shape = tuple(int(grid9[3 * d + 2]) for d in range(3))
+    Ud = _lib.to_dev(U)
+    Vd = _lib.to_dev(V)
+    if tuple(Ud.shape) != shape or tuple(Vd.shape) != shape:
+        raise ValueError(f"velocity arrays must have shape {shape}")
+    out = ctypes.c_void_p()
+    _lib.check(_lib.load().nbx_flow_create_linear3(
+        grid9.ctypes.data, _lib.ptr(Ud), _lib.ptr(Vd), int(spherical), _lib.EXTRAP[extrap_mode],
+        float(r), ctypes.byref(out), _lib.stream_ptr()))
+    return FlowHandle(out.value, _lib.NBX_FLOW_LINEAR3, grid=grid_vel)
+
+
+def ingest_xy(U_xy, scale=1.0):
+    """Velocity samples stored 'xy' (nt, ny, nx) -> 'ij' (nt, nx, ny) times
+    scale, on the GPU (the np.moveaxis(u, 1, 2) * 3.6 of the reference's
+    plot_merra_ftle.py:37-41; scale=3.6 converts m/s to km/h).  Returns a
+    device tensor, ready for prefilter3_device / get_flow_linear_2D."""
+    Ud = _lib.to_dev(U_xy)
+    if Ud.ndim != 3:
+        raise ValueError("U_xy must be a (nt, ny, nx) array")
+    nt, ny, nx = (int(v) for v in Ud.shape)
+    out = _lib.empty((nt, nx, ny))
+    _lib.check(_lib.load().nbx_ingest_xy(_lib.ptr(Ud), nt, ny, nx, float(scale), _lib.ptr(out),
+                                         _lib.stream_ptr()))
+    return out
 
 
 def flow_velocity(funcptr, t, pts, params):

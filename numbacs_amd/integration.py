@@ -48,8 +48,8 @@ def flowmap_grid_2d_device(funcptr, t0, T, xd, yd, params, rtol=1e-6, atol=1e-8,
                            status=None, nsteps=None):
     """Device-tensor form: xd (nx,), yd (ny,) float64 cuda tensors; returns
     the (nx, ny, 2) flow-map tensor (enqueued on the current stream).
-    precision: 0 (NBX_FP64) or 1 (NBX_FP32_DISP: float stages and spline,
-    double time, step control and anchor)."""
+    precision: 0 (NBX_FP64) or 1 (NBX_FP32_DISP: float stages, step
+    displacements and spline; double position, time and step control)."""
     _check_flow(funcptr)
     p = _params(params)
     nx, ny = int(xd.shape[0]), int(yd.shape[0])

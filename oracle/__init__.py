@@ -17,7 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libnbx_oracle.so")
 
-KIND = {"double_gyre": 0, "bickley_jet": 1, "spline": 2}
+KIND = {"double_gyre": 0, "bickley_jet": 1, "spline": 2, "linear": 3}
 EXTRAP = {"constant": 0, "linear": 1, "nearest": 2}
 
 _lib = None
@@ -55,6 +55,8 @@ def lib():
         L.orc_prefilter2.restype = None
         L.orc_spline3_eval_pts.argtypes = [_p, _p, _i, _p, _l, _p]
         L.orc_spline3_eval_pts.restype = None
+        L.orc_linear3_eval_pts.argtypes = [_p, _p, _i, _p, _l, _p]
+        L.orc_linear3_eval_pts.restype = None
         L.orc_ftle_grid_2d.argtypes = [_p, _i, _i, _d, _d, _d, _p, _p]
         L.orc_ftle_grid_2d.restype = None
         L.orc_C_eig_2d.argtypes = [_p, _i, _i, _d, _d, _p, _p, _p]
@@ -100,9 +102,14 @@ class Spline:
         self.r = float(r)
 
 
+class Linear(Spline):
+    """Trilinear flow description for the oracle (mirrors get_flow_linear_2D,
+    flows.py:418-506): U, V are the raw (nt, nx, ny) samples."""
+
+
 def _flow_args(kind, params, spline):
     params = _f64(params)
-    if kind == "spline":
+    if kind in ("spline", "linear"):
         s = spline
         return (KIND[kind], params, s.grid9, s.Cu, s.Cv, s.spherical, s.extrap, s.r)
     return (KIND[kind], params, None, None, None, 0, 0, 0.0)
@@ -229,6 +236,17 @@ def spline3_eval(grid, C, pts, extrap_mode="constant"):
     n = pts.shape[0]
     out = np.zeros(n)
     lib().orc_spline3_eval_pts(_ptr(grid9), _ptr(C), EXTRAP[extrap_mode], _ptr(pts), n, _ptr(out))
+    return out
+
+
+def linear3_eval(grid, V, pts, extrap_mode="constant"):
+    """Trilinear interpolant of V (nt, nx, ny) at pts (n, 3) = (t, x, y)."""
+    grid9 = np.array([v for ax in grid for v in ax], dtype=np.float64)
+    V = _f64(V)
+    pts = _f64(pts)
+    out = np.zeros(len(pts))
+    lib().orc_linear3_eval_pts(_ptr(grid9), _ptr(V), EXTRAP[extrap_mode], _ptr(pts), len(pts),
+                               _ptr(out))
     return out
 
 

@@ -170,15 +170,15 @@ static void rhs_bickley(const void* ctx, double t, const double* y, double* dy)
                       p[5] * p[8] * sin(p[8] * (y[0] - p[11] * tt))));
 }
 
-/* Tricubic natural B-spline on a uniform (t, x, y) grid.  Coefficient arrays
- * have shape (nt+2, nx+2, ny+2), C-order.  Extrapolation (parity unpinned --
- * the reference's dependency is not importable and no reference test leaves
- * the grid):
- *   0 "constant": the query point is clamped into the grid box per axis, so
- *                 outside values equal the spline on the nearest box face;
- *   1 "linear"  : value at the clamped point plus the spline gradient there
- *                 times the outward offset (first-order continuation);
- *   2 "nearest" : same as "constant".                                       */
+/* Tricubic spline and trilinear interpolant over a uniform (t, x, y) grid.
+ * Out-of-box queries by extrap mode:
+ *   0 "constant": 0 (the interpolation package's CONSTANT rule, pinned for
+ *                 eval_linear by the composition fixture fm_ci.npy);
+ *   1 "linear"  : spline: value at the clamped point plus the gradient there
+ *                 times the outward offset; trilinear: the boundary cell's
+ *                 trilinear function continued (weights outside [0, 1]);
+ *   2 "nearest" : the query clamped into the box per axis.
+ * Modes 1 and 2 are not pinned by the reference (no test leaves the grid). */
 typedef struct {
     double lo[3], hi[3];
     int n[3]; /* data points per axis (coefficients are n+2) */
@@ -286,23 +286,66 @@ static double py_fmod(double a, double m)
     return r;
 }
 
-/* flows.py:154-254 */
-static void rhs_spline(const void* ctx, double t, const double* y, double* dy)
+/* eval_linear on a uniform 3-D grid (values (n0, n1, n2), last index
+ * fastest): cell index floor((q - lo)/step) clamped to [0, n-2], weights
+ * q/step - index; out-of-box queries by extrap mode (see above).          */
+double orc_linear3_eval(const orc_spline* g, const double* V, double t, double x, double y)
 {
-    const orc_spline* g = (const orc_spline*)ctx;
+    double q[3] = {t, x, y};
+    int out = 0;
+    for (int d = 0; d < 3; ++d) out |= (q[d] < g->lo[d] || q[d] > g->hi[d]);
+    if (out && g->extrap == 0) return 0.0;
+    int i[3];
+    double w[3];
+    for (int d = 0; d < 3; ++d) {
+        double qq = q[d];
+        if (g->extrap == 2) qq = qq < g->lo[d] ? g->lo[d] : (qq > g->hi[d] ? g->hi[d] : qq);
+        double u = (qq - g->lo[d]) / ((g->hi[d] - g->lo[d]) / (g->n[d] - 1));
+        int k = (int)floor(u);
+        if (k > g->n[d] - 2) k = g->n[d] - 2;
+        if (k < 0) k = 0;
+        i[d] = k;
+        w[d] = u - k;
+    }
+    const long s1 = g->n[2], s0 = (long)g->n[1] * g->n[2];
+    const double* c = V + i[0] * s0 + i[1] * s1 + i[2];
+    double b0 = (1 - w[1]) * ((1 - w[2]) * c[0] + w[2] * c[1]) +
+                w[1] * ((1 - w[2]) * c[s1] + w[2] * c[s1 + 1]);
+    c += s0;
+    double b1 = (1 - w[1]) * ((1 - w[2]) * c[0] + w[2] * c[1]) +
+                w[1] * ((1 - w[2]) * c[s1] + w[2] * c[s1 + 1]);
+    return (1 - w[0]) * b0 + w[0] * b1;
+}
+
+/* flows.py:154-254 (spline, kind 2) and flows.py:451-506 (trilinear, kind 3) */
+static void rhs_gridded(const orc_spline* g, int linear, double t, const double* y, double* dy)
+{
     double p0 = g->p0;
     double tt = p0 * t;
+    double xx = y[0], yy = y[1];
+    if (g->spherical == 1) xx = py_fmod(y[0] - 180, 360) - 180;
+    else if (g->spherical == 2) xx = py_fmod(y[0], 360);
+    double su = linear ? orc_linear3_eval(g, g->cu, tt, xx, yy)
+                       : orc_spline3_eval(g, g->cu, tt, xx, yy);
+    double sv = linear ? orc_linear3_eval(g, g->cv, tt, xx, yy)
+                       : orc_spline3_eval(g, g->cv, tt, xx, yy);
     if (g->spherical == 1 || g->spherical == 2) {
-        double xx = g->spherical == 1 ? py_fmod(y[0] - 180, 360) - 180 : py_fmod(y[0], 360);
-        double yy = y[1];
-        double su = orc_spline3_eval(g, g->cu, tt, xx, yy);
-        double sv = orc_spline3_eval(g, g->cv, tt, xx, yy);
         dy[0] = ((p0 * su) * 180) / (M_PI * g->r * cos(yy * M_PI / 180));
         dy[1] = ((p0 * sv) * 180) / (M_PI * g->r);
     } else {
-        dy[0] = p0 * orc_spline3_eval(g, g->cu, tt, y[0], y[1]);
-        dy[1] = p0 * orc_spline3_eval(g, g->cv, tt, y[0], y[1]);
+        dy[0] = p0 * su;
+        dy[1] = p0 * sv;
     }
+}
+
+static void rhs_spline(const void* ctx, double t, const double* y, double* dy)
+{
+    rhs_gridded((const orc_spline*)ctx, 0, t, y, dy);
+}
+
+static void rhs_linear(const void* ctx, double t, const double* y, double* dy)
+{
+    rhs_gridded((const orc_spline*)ctx, 1, t, y, dy);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -462,7 +505,8 @@ static void flow_select(const orc_flow* fl, orc_rhs_fn* f, const void** ctx)
 {
     if (fl->kind == 0) { *f = rhs_double_gyre; *ctx = fl->params; }
     else if (fl->kind == 1) { *f = rhs_bickley; *ctx = fl->params; }
-    else { *f = rhs_spline; *ctx = &fl->spl; }
+    else if (fl->kind == 2) { *f = rhs_spline; *ctx = &fl->spl; }
+    else { *f = rhs_linear; *ctx = &fl->spl; }
 }
 
 static void flow_init(orc_flow* fl, int kind, const double* params, const double* grid9,
@@ -471,7 +515,7 @@ static void flow_init(orc_flow* fl, int kind, const double* params, const double
     memset(fl, 0, sizeof(*fl));
     fl->kind = kind;
     fl->params = params;
-    if (kind == 2) {
+    if (kind == 2 || kind == 3) {
         for (int d = 0; d < 3; ++d) {
             fl->spl.lo[d] = grid9[3 * d + 0];
             fl->spl.hi[d] = grid9[3 * d + 1];
@@ -821,6 +865,21 @@ void orc_spline3_eval_pts(const double* grid9, const double* C, int extrap, cons
     g.extrap = extrap;
     for (long k = 0; k < n; ++k)
         out[k] = orc_spline3_eval(&g, C, pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
+}
+
+void orc_linear3_eval_pts(const double* grid9, const double* V, int extrap, const double* pts,
+                          long n, double* out)
+{
+    orc_spline g;
+    memset(&g, 0, sizeof(g));
+    for (int d = 0; d < 3; ++d) {
+        g.lo[d] = grid9[3 * d];
+        g.hi[d] = grid9[3 * d + 1];
+        g.n[d] = (int)grid9[3 * d + 2];
+    }
+    g.extrap = extrap;
+    for (long k = 0; k < n; ++k)
+        out[k] = orc_linear3_eval(&g, V, pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
 }
 
 /* ------------------------------------------------------------------------ */

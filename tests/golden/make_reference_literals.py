@@ -9,6 +9,7 @@ numeric literals are kept:
   tests/test_flows.py:68-163   prefilter coefficients (get_interp_arrays_2D,
                                get_interp_arrays_2D_steady)
   tests/test_flows.py:208-258  tricubic evaluation at 27 in-grid points
+  tests/test_flows.py:261-285  trilinear evaluation at the same 27 points
   tests/test_flows.py:307-357  double-gyre / Bickley velocities at 27 points
   tests/test_utils.py          gradF_stencil_2D and eigvalsh_max_2D values
 """
@@ -60,6 +61,8 @@ def main():
     d["prefilter2_Cv"] = a["Cv_expected"]
     a = _assignments(tf, "test_get_callable_2D", ("veli_expected",))
     d["spline3_vel"] = a["veli_expected"]
+    a = _assignments(tf, "test_get_callable_linear_2D", ("veli_expected",))
+    d["linear3_vel"] = a["veli_expected"]
     a = _assignments(tf, "test_get_predefined_callable_dg", ("veli_expected", "domain_expected"))
     d["dg_vel"] = a["veli_expected"]
     d["dg_domain"] = a["domain_expected"]

@@ -432,3 +432,71 @@ def test_ridges_empty_and_degenerate(nb):
     assert ftle_ridge_pts(f, ev, x, y).shape == (0, 2)
     assert ftle_ridges(f, ev, x, y) == []
     assert ftle_ridges(f, ev, x, y, percentile=90) == []
+
+
+# ----------------------------------------------------- trilinear / ingest --
+def test_trilinear_flow_literals_and_oracle(nb, literals):
+    """get_flow_linear_2D at the reference test's 27 points
+    (tests/test_flows.py:261-285) and against the oracle bit for bit (both
+    evaluate the interpolant in one source order without contraction)."""
+    from numbacs_amd.flows import flow_velocity, get_flow_linear_2D
+
+    t, x, y = np.array([0.0, 0.1, 0.2]), np.array([0.0, 0.5, 1.0]), np.array([0.0, 0.5, 1.0])
+    T_, X, Y = np.meshgrid(t, x, y, indexing="ij")
+    u = np.sin(X) * np.cos(Y) + np.sin(T_)
+    v = np.cos(X) * np.sin(Y) + np.cos(T_)
+    grid = ((0.0, 0.2, 3), (0.0, 1.0, 3), (0.0, 1.0, 3))
+    ti, xi = np.array([0.05, 0.12, 0.18]), np.array([0.1, 0.4, 0.7])
+    Ti, Xi, Yi = np.meshgrid(ti, xi, xi, indexing="ij")
+    f = get_flow_linear_2D(grid, u, v)
+    vel = flow_velocity(f, Ti.ravel(), np.column_stack((Xi.ravel(), Yi.ravel())), [1.0])
+    assert np.allclose(vel.T, literals["linear3_vel"])
+    rng = np.random.default_rng(8)
+    for extrap in ("constant", "linear", "nearest"):
+        f = get_flow_linear_2D(grid, u, v, extrap_mode=extrap)
+        tq = rng.uniform(-0.05, 0.25, 400)
+        pq = np.column_stack((rng.uniform(-0.2, 1.2, 400), rng.uniform(-0.2, 1.2, 400)))
+        got = flow_velocity(f, tq, pq, [-1.0])
+        ref = oracle.rhs_eval("linear", np.array([-1.0]), tq, pq,
+                              spline=oracle.Linear(grid, u, v, extrap_mode=extrap))
+        assert np.array_equal(got, ref), extrap
+
+
+def test_trilinear_flowmap_vs_oracle(nb):
+    """A spherical trilinear flow over the synthetic field: GPU flow maps
+    (both precision forms) against the FP64 oracle."""
+    from numbacs_amd import _lib
+    from numbacs_amd.flows import get_flow_linear_2D
+    from numbacs_amd.integration import flowmap_grid_2D, flowmap_grid_2d_device
+    from numbacs_amd.workloads import synthetic_velocity
+
+    t, lon, lat, U, V = synthetic_velocity(nt=30)
+    grid = ((t[0], t[-1], len(t)), (lon[0], lon[-1], len(lon)), (lat[0], lat[-1], len(lat)))
+    f = get_flow_linear_2D(grid, U, V, spherical=1)
+    x, y = np.linspace(-90, 20, 56), np.linspace(0, 40, 41)
+    fm = flowmap_grid_2D(f, 20.0, -12.0, x, y, [-1.0])
+    L = oracle.Linear(grid, U, V, spherical=1)
+    _, st, _ = oracle.flowmap_grid_2d("linear", np.array([-1.0]), 20.0, -12.0, x, y, spline=L)
+    assert (st == 1).all()
+    # the trilinear RHS has kinks at cell faces, where the step controller's
+    # accept/reject decisions follow rounding: the conditioning bar applies
+    ref, sens = ic_sensitivity(lambda xx, yy: oracle.flowmap_grid_2d(
+        "linear", np.array([-1.0]), 20.0, -12.0, xx, yy, spline=L)[0], x, y)
+    ok, frac, err = flowmap_agreement(fm, ref, sens)
+    assert ok, float(err.max())
+    assert frac > 0.95, frac  # measured 97.5% within the plain bar
+    f32 = _lib.to_host(flowmap_grid_2d_device(
+        f, 20.0, -12.0, _lib.to_dev(x), _lib.to_dev(y), [-1.0], precision=_lib.NBX_FP32_DISP))
+    assert np.mean(np.abs(f32 - ref).max(-1) <= 1e-3) > 0.95
+
+
+def test_ingest_xy(nb):
+    """'xy' -> 'ij' transpose with the m/s -> km/h factor, exactly as
+    np.moveaxis(u, 1, 2) * 3.6 (plot_merra_ftle.py:37-41)."""
+    from numbacs_amd import _lib
+    from numbacs_amd.flows import ingest_xy
+
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal((5, 7, 11))
+    got = _lib.to_host(ingest_xy(u, 3.6))
+    assert np.array_equal(got, np.moveaxis(u, 1, 2) * 3.6)

@@ -371,3 +371,48 @@ def test_composition_of_linear_maps_is_exact():
     one = oracle.flowmap_composition(F[:1], grid, 1)
     a, c, b, d = maps[0][0], maps[0][2], maps[0][0], maps[0][3]
     assert np.abs(one - np.stack([a * (a * X + c) + c, b * (b * Y + d) + d], -1)).max() < 1e-13
+
+
+# ------------------------------------------------------- trilinear flows --
+def _coords_2d():
+    t, x, y = np.array([0.0, 0.1, 0.2]), np.array([0.0, 0.5, 1.0]), np.array([0.0, 0.5, 1.0])
+    T_, X, Y = np.meshgrid(t, x, y, indexing="ij")
+    u = np.sin(X) * np.cos(Y) + np.sin(T_)
+    v = np.cos(X) * np.sin(Y) + np.cos(T_)
+    ti, xi = np.array([0.05, 0.12, 0.18]), np.array([0.1, 0.4, 0.7])
+    Ti, Xi, Yi = np.meshgrid(ti, xi, xi, indexing="ij")
+    pts = np.column_stack((Ti.ravel(), Xi.ravel(), Yi.ravel()))
+    return ((0.0, 0.2, 3), (0.0, 1.0, 3), (0.0, 1.0, 3)), u, v, pts
+
+
+def test_trilinear_literals(literals):
+    """tests/test_flows.py:261-285 of the reference (get_callable_linear_2D
+    at 27 in-grid points), through the interpolant and the flow RHS."""
+    grid, u, v, pts = _coords_2d()
+    want = literals["linear3_vel"]
+    assert np.allclose(oracle.linear3_eval(grid, u, pts), want[0])
+    assert np.allclose(oracle.linear3_eval(grid, v, pts), want[1])
+    rhs = oracle.rhs_eval("linear", np.array([1.0]), pts[:, 0], pts[:, 1:],
+                          spline=oracle.Linear(grid, u, v))
+    assert np.allclose(rhs.T, want)
+
+
+def test_trilinear_exact_and_extrapolation():
+    """Fields linear in each variable are reproduced to rounding in the box;
+    outside it constant gives 0, nearest the clamped value, linear the
+    field itself (a multilinear field continues its boundary cell)."""
+    t, x, y = np.linspace(0, 3, 7), np.linspace(-1, 1, 9), np.linspace(0, 2, 8)
+    T_, X, Y = np.meshgrid(t, x, y, indexing="ij")
+
+    def f(a, b, c):
+        return 0.3 * a - 1.7 * b + 2.5 * c + 0.25 + 0.5 * a * b - 0.2 * b * c
+
+    F = f(T_, X, Y)
+    grid = ((0.0, 3.0, 7), (-1.0, 1.0, 9), (0.0, 2.0, 8))
+    rng = np.random.default_rng(4)
+    q = np.column_stack((rng.uniform(0, 3, 300), rng.uniform(-1, 1, 300), rng.uniform(0, 2, 300)))
+    assert np.abs(oracle.linear3_eval(grid, F, q) - f(*q.T)).max() < 1e-12
+    qo = q + np.array([0.0, 0.0, 2.5])
+    assert np.array_equal(oracle.linear3_eval(grid, F, qo, "constant"), np.zeros(len(q)))
+    assert np.abs(oracle.linear3_eval(grid, F, qo, "nearest") - f(q[:, 0], q[:, 1], 2.0)).max() < 1e-12
+    assert np.abs(oracle.linear3_eval(grid, F, qo, "linear") - f(*qo.T)).max() < 1e-11
