@@ -500,3 +500,32 @@ def test_ingest_xy(nb):
     u = rng.standard_normal((5, 7, 11))
     got = _lib.to_host(ingest_xy(u, 3.6))
     assert np.array_equal(got, np.moveaxis(u, 1, 2) * 3.6)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_particles_stop_at_grid_edge(nb, precision):
+    """Zero velocity outside the grid ('constant' extrapolation): in a
+    uniform westward field every particle runs to x = 0 and stops there,
+    x(T) = max(x0 - 0.5 T, 0) up to the step control's resolution of the
+    velocity jump, with a bounded number of steps in both precision forms
+    (a float displacement from the start point once stalled here, its
+    increments falling below float resolution as the steps shrank)."""
+    from numbacs_amd import _lib
+    from numbacs_amd.flows import get_flow_2D, get_interp_arrays_2D
+    from numbacs_amd.integration import flowmap_grid_2d_device
+
+    t, x, y = np.linspace(0, 48, 25), np.linspace(0, 10, 21), np.linspace(0, 5, 11)
+    U = np.full((25, 21, 11), -0.5)
+    V = np.zeros((25, 21, 11))
+    grid, Cu, Cv = get_interp_arrays_2D(t, x, y, U, V)
+    f = get_flow_2D(grid, Cu, Cv)
+    x0, y0 = np.linspace(0.3, 9.7, 95), np.linspace(0.5, 4.5, 9)
+    st = _lib.empty((95, 9), "i32")
+    ns = _lib.empty((95, 9), "i32")
+    fm = _lib.to_host(flowmap_grid_2d_device(f, 0.0, 40.0, _lib.to_dev(x0), _lib.to_dev(y0),
+                                             [1.0], precision=precision, status=st, nsteps=ns))
+    assert (_lib.to_host(st) == 1).all()
+    assert _lib.to_host(ns).max() < 1000
+    want = np.maximum(x0 - 0.5 * 40.0, 0.0)
+    assert np.abs(fm[..., 0] - want[:, None]).max() < 1e-3
+    assert np.abs(fm[..., 1] - y0[None, :]).max() < 1e-12
