@@ -260,6 +260,34 @@ int nbx_ridges(const double* f, const double* eigvec_max, const double* x, int n
 int nbx_label8(const uint8_t* b, int nx, int ny, int32_t* labels, int32_t* nlabels_host,
                nbx_stream stream);
 
+/* ftle_ordered_ridges (ridges.py:720-1054): ridge points (the ftle_ridge_pts
+ * predicate, percentile and sdd_thresh as there) linked into ordered curves
+ * from the strongest second directional derivative (_linked_ridge_pts,
+ * ridges.py:418-603), then curves with endpoints within dist_tol whose
+ * tangents agree within ep_tan_ang joined; curves shorter than
+ * min_ridge_pts dropped.  The predicate runs on the GPU (device f, evec,
+ * x, y); the serial linking runs on the host.  Output to HOST buffers:
+ * ridge k is points [offsets[k], offsets[k+1]) of pts (x/y pairs).  Fails
+ * if pts_cap (points) or offsets_cap are too small, with *npts_host and
+ * *nridges_host set to the sizes needed (nx*ny points always suffice). */
+int nbx_ordered_ridges(const double* f, const double* eigvec_max, const double* x, int nx,
+                       const double* y, int ny, double dist_tol, double ep_tan_ang,
+                       int min_ridge_pts, double sdd_thresh, double percentile, double c,
+                       double* pts_host, int64_t pts_cap, int64_t* offsets_host,
+                       int64_t offsets_cap, int64_t* nridges_host, int64_t* npts_host,
+                       nbx_stream stream);
+
+/* The host half of nbx_ordered_ridges on caller-provided HOST buffers (no
+ * GPU needed): pts_dense (nx*ny, 2) ridge points, eigvec_max (nx*ny, 2),
+ * sdd (nx*ny) second directional derivative (< 0 exactly at ridge points,
+ * which must lie 2+ points inside the grid, 0 elsewhere), h = min(dx, dy).
+ * Outputs as nbx_ordered_ridges. */
+int nbx_ordered_ridges_link(const double* pts_dense, const double* eigvec_max, const double* sdd,
+                            int nx, int ny, double h, double dist_tol, double ep_tan_ang,
+                            int min_ridge_pts, double sdd_thresh, double c, double* pts_host,
+                            int64_t pts_cap, int64_t* offsets_host, int64_t offsets_cap,
+                            int64_t* nridges_host, int64_t* npts_host);
+
 /* --- flow-map surrogate (BF16 MFMA MLP) --------------------------------- */
 /* No reference counterpart: kernels of the learned flow-map surrogate that
  * the FTLE path may use with integrator fallback.  bf16 tensors are raw

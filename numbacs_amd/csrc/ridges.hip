@@ -27,6 +27,7 @@
 #include <cstring>
 
 #include "common.h"
+#include "ordered_ridges.h"
 
 namespace nbx {
 
@@ -180,12 +181,14 @@ int percentile_device(const double* f, int64_t n, double q, SelState* st, double
 __global__ __launch_bounds__(256) void k_ridge_predicate(
     const double* __restrict__ f, const double2* __restrict__ evec, const double* __restrict__ xs,
     const double* __restrict__ ys, int nx, int ny, double dx, double dy, double sdd_thresh,
-    const double* __restrict__ fmin_dev, int32_t* __restrict__ flag, double2* __restrict__ pts) {
+    const double* __restrict__ fmin_dev, int32_t* __restrict__ flag, double2* __restrict__ pts,
+    double* __restrict__ sdd_out) {
     const int j = blockIdx.x * 64 + threadIdx.x;
     const int i = blockIdx.y * 4 + threadIdx.y;
     if (i >= nx || j >= ny) return;
     const long k = static_cast<long>(i) * ny + j;
     int32_t hit = 0;
+    double sdd = 0.0;
     if (i >= 2 && i < nx - 2 && j >= 2 && j < ny - 2) {
         const double f_min = fmin_dev ? *fmin_dev : 0.0;
         const double f0 = f[k];
@@ -205,11 +208,13 @@ __global__ __launch_bounds__(256) void k_ridge_predicate(
                 if (fabs(t * ex) <= dx / 2 && fabs(t * ey) <= dy / 2) {
                     pts[k] = make_double2(xs[i] + t * ex, ys[j] + t * ey);
                     hit = 1;
+                    sdd = c2;
                 }
             }
         }
     }
     flag[k] = hit;
+    if (sdd_out != nullptr) sdd_out[k] = sdd;
 }
 
 // ---------------------------------------------------------------------------
@@ -556,7 +561,8 @@ size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 // Ridge predicate + optional percentile; leaves flag/pts_dense in the arena.
 int run_predicate(const double* f, const double* evec, const double* x, const double* y, int nx,
                   int ny, double dx, double dy, double sdd_thresh, double percentile,
-                  SelState* st, int32_t* flag, double2* pts_dense, hipStream_t s) {
+                  SelState* st, int32_t* flag, double2* pts_dense, hipStream_t s,
+                  double* sdd = nullptr) {
     const int64_t n = static_cast<int64_t>(nx) * ny;
     const double* fmin_dev = nullptr;
     if (percentile != 0) {
@@ -566,8 +572,29 @@ int run_predicate(const double* f, const double* evec, const double* x, const do
     }
     const dim3 block(64, 4), grid((ny + 63) / 64, (nx + 3) / 4);
     k_ridge_predicate<<<grid, block, 0, s>>>(f, reinterpret_cast<const double2*>(evec), x, y, nx,
-                                             ny, dx, dy, sdd_thresh, fmin_dev, flag, pts_dense);
+                                             ny, dx, dy, sdd_thresh, fmin_dev, flag, pts_dense,
+                                             sdd);
     NBX_LAUNCHED("ridge_predicate");
+    return 0;
+}
+
+// Host linking and joining shared by both ordered-ridge entry points.
+int link_and_join(const ordered::RidgeField& F, double dist_tol, double ep_tan_ang,
+                  int min_ridge_pts, double sdd_thresh, double c, double* pts_host,
+                  int64_t pts_cap, int64_t* offsets_host, int64_t offsets_cap,
+                  int64_t* nridges_host, int64_t* npts_host) {
+    const ordered::Linked L = ordered::link(F, sdd_thresh, c);
+    std::vector<double> out;
+    std::vector<int64_t> offsets;
+    ordered::join(L, dist_tol, ep_tan_ang, min_ridge_pts, out, offsets);
+    const int64_t nr = static_cast<int64_t>(offsets.size()) - 1;
+    const int64_t np_ = static_cast<int64_t>(out.size() / 2);
+    *nridges_host = nr;
+    *npts_host = np_;
+    NBX_REQUIRE(np_ <= pts_cap && nr + 1 <= offsets_cap,
+                "output buffers too small (the counts hold the sizes needed)");
+    std::copy(out.begin(), out.end(), pts_host);
+    std::copy(offsets.begin(), offsets.end(), offsets_host);
     return 0;
 }
 
@@ -772,6 +799,102 @@ int nbx_label8(const uint8_t* b, int nx, int ny, int32_t* labels, int32_t* nlabe
     NBX_HIP(hipStreamSynchronize(s));
     *nlabels_host = cnt;
     return 0;
+}
+
+int nbx_ordered_ridges(const double* f, const double* evec, const double* x, int nx,
+                       const double* y, int ny, double dist_tol, double ep_tan_ang,
+                       int min_ridge_pts, double sdd_thresh, double percentile, double c,
+                       double* pts_host, int64_t pts_cap, int64_t* offsets_host,
+                       int64_t offsets_cap, int64_t* nridges_host, int64_t* npts_host,
+                       nbx_stream stream) {
+    if (int rc = check_grid(nx, ny)) return rc;
+    NBX_REQUIRE(nridges_host != nullptr && npts_host != nullptr, "NULL count pointer");
+    *nridges_host = 0;
+    *npts_host = 0;
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    if (n == 0) return 0;
+    NBX_REQUIRE(nx >= 2 && ny >= 2, "the ridge stencil needs at least 2 points per axis");
+    NBX_REQUIRE(f && evec && x && y && pts_host && offsets_host, "NULL array");
+    hipStream_t s = as_stream(stream);
+    const size_t bytes = align256(sizeof(SelState)) + align256(4 * n) + align256(16 * n) +
+                         align256(8 * n);
+    int err = 0;
+    Arena A{static_cast<char*>(workspace(1, bytes, &err)), 0};
+    if (err) return err;
+    auto* st = A.take<SelState>(1);
+    auto* flag = A.take<int32_t>(n);
+    auto* dense = A.take<double2>(n);
+    auto* sdd = A.take<double>(n);
+    double xh[2], yh[2];
+    NBX_HIP(hipMemcpyAsync(xh, x, sizeof(xh), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipMemcpyAsync(yh, y, sizeof(yh), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    const double dx = xh[1] - xh[0], dy = yh[1] - yh[0];
+    if (int rc = run_predicate(f, evec, x, y, nx, ny, dx, dy, sdd_thresh, percentile, st, flag,
+                               dense, s, sdd))
+        return rc;
+    // the serial linking runs on the host over the dense per-point buffers
+    std::vector<int32_t> fl(n);
+    std::vector<double2> pt(n), ev(n);
+    ordered::RidgeField F;
+    F.nx = nx;
+    F.ny = ny;
+    F.h = dx < dy ? dx : dy;
+    F.sdd.resize(n);
+    NBX_HIP(hipMemcpyAsync(fl.data(), flag, 4 * n, hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipMemcpyAsync(pt.data(), dense, 16 * n, hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipMemcpyAsync(ev.data(), evec, 16 * n, hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipMemcpyAsync(F.sdd.data(), sdd, 8 * n, hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    F.px.resize(n);
+    F.py.resize(n);
+    F.vx.resize(n);
+    F.vy.resize(n);
+    for (int64_t k = 0; k < n; ++k) {
+        F.px[k] = fl[k] ? pt[k].x : -1.0;
+        F.py[k] = fl[k] ? pt[k].y : -1.0;
+        F.vx[k] = ev[k].x;
+        F.vy[k] = ev[k].y;
+    }
+    return link_and_join(F, dist_tol, ep_tan_ang, min_ridge_pts, sdd_thresh, c, pts_host, pts_cap,
+                         offsets_host, offsets_cap, nridges_host, npts_host);
+}
+
+int nbx_ordered_ridges_link(const double* pts_dense, const double* eigvec_max, const double* sdd,
+                            int nx, int ny, double h, double dist_tol, double ep_tan_ang,
+                            int min_ridge_pts, double sdd_thresh, double c, double* pts_host,
+                            int64_t pts_cap, int64_t* offsets_host, int64_t offsets_cap,
+                            int64_t* nridges_host, int64_t* npts_host) {
+    if (int rc = check_grid(nx, ny)) return rc;
+    NBX_REQUIRE(nridges_host != nullptr && npts_host != nullptr, "NULL count pointer");
+    *nridges_host = 0;
+    *npts_host = 0;
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    if (n == 0) return 0;
+    NBX_REQUIRE(pts_dense && eigvec_max && sdd && pts_host && offsets_host, "NULL array");
+    NBX_REQUIRE(h > 0, "grid spacing must be positive");
+    ordered::RidgeField F;
+    F.nx = nx;
+    F.ny = ny;
+    F.h = h;
+    F.px.resize(n);
+    F.py.resize(n);
+    F.vx.resize(n);
+    F.vy.resize(n);
+    F.sdd.assign(sdd, sdd + n);
+    for (int64_t k = 0; k < n; ++k) {
+        F.px[k] = pts_dense[2 * k];
+        F.py[k] = pts_dense[2 * k + 1];
+        F.vx[k] = eigvec_max[2 * k];
+        F.vy[k] = eigvec_max[2 * k + 1];
+    }
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            NBX_REQUIRE(F.sdd[static_cast<int64_t>(i) * ny + j] >= 0 ||
+                            (i >= 2 && i < nx - 2 && j >= 2 && j < ny - 2),
+                        "ridge points (sdd < 0) must lie two or more points inside the grid");
+    return link_and_join(F, dist_tol, ep_tan_ang, min_ridge_pts, sdd_thresh, c, pts_host, pts_cap,
+                         offsets_host, offsets_cap, nridges_host, npts_host);
 }
 
 }  // extern "C"

@@ -1,7 +1,7 @@
 """FTLE ridge extraction (mirrors numbacs.extraction.ridges).
 
-Reference: ftle_ridge_pts (ridges.py:9-76) and ftle_ridges
-(ridges.py:186-229).  The percentile threshold, ridge predicate, compaction,
+Reference: ftle_ridge_pts (ridges.py:9-76), ftle_ridges
+(ridges.py:186-229) and ftle_ordered_ridges (ridges.py:720-1054).  The percentile threshold, ridge predicate, compaction,
 8-connected labelling and grouping all run on the GPU; the host only splits
 the grouped point buffer into the list the reference returns.
 """
@@ -99,3 +99,26 @@ def label8(b):
     _lib.check(_lib.load().nbx_label8(_lib.ptr(bd), nx, ny, _lib.ptr(lab), ctypes.byref(n),
                                       _lib.stream_ptr()))
     return _lib.to_host(lab), n.value
+
+
+def ftle_ordered_ridges(f, eigvec_max, x, y, dist_tol, ep_tan_ang=np.pi / 4, min_ridge_pts=5,
+                        sdd_thresh=0.0, percentile=0, c=1.0):
+    """Ordered FTLE ridge curves (ridges.py:720-1054): ridge points linked
+    along the ridge tangent from the strongest point outward, then curves
+    joined where their endpoints lie within dist_tol and their tangents agree
+    within ep_tan_ang; curves shorter than min_ridge_pts are dropped.
+    Returns a list of (n_k, 2) arrays of ordered points.  The ridge predicate
+    and percentile run on the GPU, the serial linking in native host code."""
+    fd, evd, xd, yd, nx, ny, _, _ = _inputs(f, eigvec_max, x, y)
+    n = nx * ny
+    if n == 0:
+        return []
+    pts = np.zeros((n, 2))
+    offsets = np.zeros(n + 1, np.int64)
+    nr, npt = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(_lib.load().nbx_ordered_ridges(
+        _lib.ptr(fd), _lib.ptr(evd), _lib.ptr(xd), nx, _lib.ptr(yd), ny, float(dist_tol),
+        float(ep_tan_ang), int(min_ridge_pts), float(sdd_thresh), float(percentile), float(c),
+        pts.ctypes.data, n, offsets.ctypes.data, n + 1, ctypes.byref(nr), ctypes.byref(npt),
+        _lib.stream_ptr()))
+    return [pts[offsets[k]:offsets[k + 1]].copy() for k in range(nr.value)]
