@@ -467,6 +467,8 @@ int bitonic_sort(unsigned long long* keys, int64_t N, hipStream_t s) {
     return 0;
 }
 
+__global__ void k_offsets_zero(int64_t* __restrict__ offsets) { offsets[0] = 0; }
+
 __global__ void k_make_keys(const int32_t* __restrict__ idx, const int32_t* __restrict__ par,
                             const int32_t* __restrict__ count_dev, int64_t N,
                             unsigned long long* __restrict__ keys) {
@@ -730,23 +732,38 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     if (int rc = exclusive_scan(flag, n, pos, tmp, count, s)) return rc;
     k_compact<<<nb, 256, 0, s>>>(flag, pos, dense, n, nullptr, idx);
     NBX_LAUNCHED("compact");
+    // The grouping stage works on the ridge points only (typically a few
+    // percent of the grid): read their count, then sort and segment that many.
+    int32_t cnt = 0;
+    NBX_HIP(hipMemcpyAsync(&cnt, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    if (cnt == 0) {
+        k_offsets_zero<<<1, 1, 0, s>>>(offsets);
+        NBX_LAUNCHED("offsets_zero");
+        NBX_HIP(hipStreamSynchronize(s));
+        return 0;
+    }
+    const int64_t m = cnt;
+    int64_t Ns = kSortChunk;
+    while (Ns < m) Ns <<= 1;
+    const unsigned nbm = blocks_for(m, 256);
     // order by (component root, raveled index)
-    k_make_keys<<<blocks_for(N, 256), 256, 0, s>>>(idx, par, count, N, keys);
+    k_make_keys<<<blocks_for(Ns, 256), 256, 0, s>>>(idx, par, count, Ns, keys);
     NBX_LAUNCHED("make_keys");
-    if (int rc = bitonic_sort(keys, N, s)) return rc;
+    if (int rc = bitonic_sort(keys, Ns, s)) return rc;
     // segments = components
-    k_seg_start<<<nb, 256, 0, s>>>(keys, count, n, seg_start);
+    k_seg_start<<<nbm, 256, 0, s>>>(keys, count, m, seg_start);
     NBX_LAUNCHED("seg_start");
-    if (int rc = exclusive_scan(seg_start, n, seg_ex, tmp, nseg, s)) return rc;
-    k_seg_first<<<nb, 256, 0, s>>>(seg_start, seg_ex, count, nseg, n, seg_first);
+    if (int rc = exclusive_scan(seg_start, m, seg_ex, tmp, nseg, s)) return rc;
+    k_seg_first<<<nbm, 256, 0, s>>>(seg_start, seg_ex, count, nseg, m, seg_first);
     NBX_LAUNCHED("seg_first");
-    k_seg_keep<<<nb, 256, 0, s>>>(seg_first, nseg, n, min_ridge_pts, kept_len, kept_flag);
+    k_seg_keep<<<nbm, 256, 0, s>>>(seg_first, nseg, m, min_ridge_pts, kept_len, kept_flag);
     NBX_LAUNCHED("seg_keep");
-    if (int rc = exclusive_scan(kept_len, n, out_off, tmp, total, s)) return rc;
-    if (int rc = exclusive_scan(kept_flag, n, ridge_idx, tmp, nridges, s)) return rc;
-    k_group_scatter<<<nb, 256, 0, s>>>(keys, seg_start, seg_ex, seg_first, kept_flag, out_off,
-                                       ridge_idx, dense, count, n,
-                                       reinterpret_cast<double2*>(pts), offsets);
+    if (int rc = exclusive_scan(kept_len, m, out_off, tmp, total, s)) return rc;
+    if (int rc = exclusive_scan(kept_flag, m, ridge_idx, tmp, nridges, s)) return rc;
+    k_group_scatter<<<nbm, 256, 0, s>>>(keys, seg_start, seg_ex, seg_first, kept_flag, out_off,
+                                        ridge_idx, dense, count, m,
+                                        reinterpret_cast<double2*>(pts), offsets);
     NBX_LAUNCHED("group_scatter");
     k_offsets_end<<<1, 1, 0, s>>>(nridges, total, offsets);
     NBX_LAUNCHED("offsets_end");
