@@ -415,10 +415,17 @@ struct Spline3 {
         if (spherical == 0) {
             f0 = static_cast<R>(p0 * static_cast<double>(su));
             f1 = static_cast<R>(p0 * static_cast<double>(sv));
+        } else if constexpr (sizeof(R) == sizeof(double)) {
+            f0 = ((p0 * su) * 180) / (kPi * r * cos(y1 * kPi / 180));
+            f1 = ((p0 * sv) * 180) / (kPi * r);
         } else {
-            f0 = static_cast<R>(((p0 * static_cast<double>(su)) * 180) /
-                                (kPi * r * cos(y1 * kPi / 180)));
-            f1 = static_cast<R>(((p0 * static_cast<double>(sv)) * 180) / (kPi * r));
+            // the float form converts km/h to degrees/h in float too: the
+            // velocity is already float, and the cosine of a float latitude
+            // is good to ~1e-7, below the form's own rounding
+            const float q = static_cast<float>(p0) * 180.0f;
+            const float pr = static_cast<float>(kPi * r);
+            f0 = (q * su) / (pr * cosf(static_cast<float>(y1) * static_cast<float>(kPi / 180)));
+            f1 = (q * sv) / pr;
         }
     }
 };
