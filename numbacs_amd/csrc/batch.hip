@@ -99,7 +99,10 @@ int launch_batch_flow(nbx_batch* b, hipStream_t s) {
     const double* p = b->params.data();
     const LaunchCfg cfg = launch_cfg(fl->kind, b->precision);
     if (fl->kind == NBX_FLOW_SPLINE3) {
-        const Spline3<R, 1> sp = make_spline<R>(fl, p[0]);
+        if (fl->extrap == 1)  // the gradient path, one lane per particle
+            return launch_batch_queue<Spline3<R, 1, true>, R, 1>(b, make_spline<R, true>(fl, p[0]),
+                                                                 cfg, s);
+        const Spline3<R, 1> sp = make_spline<R, false>(fl, p[0]);
         if (cfg.lanes == 4) return launch_batch_queue<Spline3<R, 4>, R, 4>(b, regroup<4>(sp), cfg, s);
         if (cfg.lanes == 2) return launch_batch_queue<Spline3<R, 2>, R, 2>(b, regroup<2>(sp), cfg, s);
         return launch_batch_queue<Spline3<R, 1>, R, 1>(b, sp, cfg, s);

@@ -38,9 +38,11 @@ struct LaunchCfg {
 
 LaunchCfg launch_cfg(int kind, int precision);
 
-template <class R>
-Spline3<R, 1> make_spline(const nbx_flow* fl, double p0) {
-    Spline3<R, 1> s;
+// L: compile the linear-extrapolation (gradient) path into the functor; the
+// dispatchers set it exactly when the flow asks for extrap 1
+template <class R, bool L>
+Spline3<R, 1, L> make_spline(const nbx_flow* fl, double p0) {
+    Spline3<R, 1, L> s;
     if constexpr (sizeof(R) == 8) s.C = fl->coef;
     else s.C = fl->coef32;
     for (int d = 0; d < 3; ++d) {
@@ -80,7 +82,7 @@ Linear3<R> make_linear(const nbx_flow* fl, double p0) {
 }
 
 template <int G, class R>
-Spline3<R, G> regroup(const Spline3<R, 1>& a) {
+Spline3<R, G> regroup(const Spline3<R, 1, false>& a) {
     Spline3<R, G> s;
     s.C = a.C;
     for (int d = 0; d < 3; ++d) {

@@ -368,7 +368,12 @@ int dispatch_solve(const nbx_flow* fl, const double* params, int np, const Src& 
                    const SolveArgs& a, int precision, hipStream_t s) {
     const LaunchCfg cfg = launch_cfg(fl->kind, precision);
     if (fl->kind == NBX_FLOW_SPLINE3) {
-        const Spline3<R, 1> sp = make_spline<R>(fl, params[0]);
+        // linear extrapolation: the kernel with the gradient path, one lane
+        // per particle (the lane-group setting is not used for such flows)
+        if (fl->extrap == 1)
+            return launch_solve<Spline3<R, 1, true>, R, 1>(make_spline<R, true>(fl, params[0]),
+                                                           src, a, cfg, s);
+        const Spline3<R, 1> sp = make_spline<R, false>(fl, params[0]);
         if (cfg.lanes == 4) return launch_solve<Spline3<R, 4>, R, 4>(regroup<4>(sp), src, a, cfg, s);
         if (cfg.lanes == 2) return launch_solve<Spline3<R, 2>, R, 2>(regroup<2>(sp), src, a, cfg, s);
         return launch_solve<Spline3<R, 1>, R, 1>(sp, src, a, cfg, s);
@@ -547,8 +552,8 @@ int nbx_flow_eval(const nbx_flow* fl, const double* params, int np, const double
     auto* O = reinterpret_cast<double2*>(out);
     const unsigned nb = blocks_for(n, 256);
     if (fl->kind == NBX_FLOW_SPLINE3) {
-        k_flow_eval<Spline3<double, 1>, double><<<nb, 256, 0, s>>>(
-            make_spline<double>(fl, params[0]), t, P, n, O);
+        k_flow_eval<Spline3<double, 1, true>, double><<<nb, 256, 0, s>>>(
+            make_spline<double, true>(fl, params[0]), t, P, n, O);
     } else if (fl->kind == NBX_FLOW_LINEAR3) {
         k_flow_eval<Linear3<double>, double><<<nb, 256, 0, s>>>(
             make_linear<double>(fl, params[0]), t, P, n, O);
