@@ -73,7 +73,11 @@ def test_flowmap_aux_edge_rules_vs_oracle(lib, eig_main, compute_edge):
 @pytest.mark.parametrize("lanes", [1, 2, 4])
 def test_flowmap_aux_spline_lane_groups(lib, lanes):
     """The aux-grid particle source under every lane-group shape of the
-    spline gathers: equal to the point-list flow map of the same starts."""
+    spline gathers: the point-list flow map of the same starts, to rounding.
+    The two entry points are separate kernel instantiations whose FMA
+    contraction may differ, so a few trajectories differ in the last bits
+    (measured 1.7e-13 / 2.7e-13 of a scale of 88 for 2 / 4 lanes); a
+    misplaced start would differ by the offset h = 0.05 and more."""
     from numbacs_amd import autotune
     from numbacs_amd.flows import get_flow_2D, get_interp_arrays_2D
     from numbacs_amd.integration import flowmap, flowmap_aux_grid_2D
@@ -93,7 +97,9 @@ def test_flowmap_aux_spline_lane_groups(lib, lanes):
         starts = np.stack([X, Y], -1)[:, :, None, :] + off[None, None]
         inner = starts[1:-1, 1:-1].reshape(-1, 2)
         pts = flowmap(f, 20.0, -12.0, inner, [-1.0])
-        assert np.array_equal(fa[1:-1, 1:-1].reshape(-1, 2), pts)
+        got = fa[1:-1, 1:-1].reshape(-1, 2)
+        assert np.abs(got - pts).max() <= 1e-12 * np.abs(pts).max()
+        assert np.mean(np.all(got == pts, axis=-1)) > 0.99
         assert not fa[0, :, :4].any() and fa[0, :, 4].all()
     finally:
         autotune.set_config(2, 0, saved)
