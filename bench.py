@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--launch", default=None, metavar="LANES,BLOCK,PERSIST",
@@ -235,22 +235,28 @@ def cpu_baseline(W, xp, yp, fm_gpu, ftle_gpu, t0, T, reps):
     cores = len(os.sched_getaffinity(0))
     S = oracle.Spline(W["grid"], _host(W["Cu"]), _host(W["Cv"]), spherical=1)
     dx, dy = xp[1] - xp[0], yp[1] - yp[0]
-    times = []
+    parts = []  # (flow map, FTLE, ridges) seconds per run
     for _ in range(reps + 1):
         t1 = time.perf_counter()
         fm, st, _ = oracle.flowmap_grid_2d("spline", np.array([-1.0]), t0, T, xp, yp, spline=S,
                                            nthreads=cores)
+        t2 = time.perf_counter()
         ev, V = oracle.C_eig_2d(fm, dx, dy)
         ftle = np.where(ev[..., 1] > 1, np.log(np.maximum(ev[..., 1], 1e-300)) / (2 * abs(T)), 0)
+        t3 = time.perf_counter()
         oracle.ridges(ftle, V[..., 1], xp, yp, percentile_=90)
-        times.append(time.perf_counter() - t1)
-    med = float(np.median(times[1:]))
+        parts.append((t2 - t1, t3 - t2, time.perf_counter() - t3))
+    runs = np.array(parts[1:])
+    med = float(np.median(runs.sum(1)))
     P = len(xp) * len(yp)
     ferr = np.abs(fm_gpu - fm).max(-1) / np.abs(fm).max()
     ftle_err = np.abs(ftle_gpu - ftle) / np.abs(ftle).max()
     cpu = {"value": P / med, "unit": "grid points/s", "cores": cores, "kind": "port",
-           "sample": f"full workload ({P} particles, flow map + FTLE + ridges), median of {reps}",
-           "seconds": med}
+           "sample": f"full workload ({P} particles, flow map + FTLE + ridges), median of {reps} "
+                     "runs after one warm-up",
+           "seconds": med,
+           "seconds_flowmap_ftle_ridges": [float(v) for v in np.median(runs, 0)],
+           "cpu_model": _cpu_model()}
     err = {"flowmap_max_rel": float(ferr.max()),
            "flowmap_frac_within_1e-6": float(np.mean(ferr <= 1e-6)),
            "ftle_max_rel": float(ftle_err.max()),
@@ -261,6 +267,17 @@ def cpu_baseline(W, xp, yp, fm_gpu, ftle_gpu, t0, T, reps):
 
 def _host(t):
     return t.detach().cpu().numpy()
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 # ---------------------------------------------------------------------------
