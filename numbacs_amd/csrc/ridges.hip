@@ -6,7 +6,8 @@
 // np.percentile (linear method) and scipy.ndimage.label with a full 3x3
 // structure.
 //
-// Device pipeline (no host round trip until the final counts):
+// Device pipeline (one host round trip before the end: the ridge-point count,
+// which sizes the grouping stage):
 //   percentile  : two radix selects (8 passes of 8 bits over order-preserving
 //                 64-bit keys) for the order statistics around the virtual
 //                 index, then numpy's two-sided lerp -- f_min stays in HBM;

@@ -431,6 +431,26 @@ def test_ridges_vs_oracle_random(nb, percentile):
         assert a.shape == b.shape and np.allclose(a, b, rtol=0, atol=1e-14)
 
 
+def test_ridges_grouping_beyond_one_workgroup(nb):
+    """28,120 ridge points on 95 wavy ridge lines: the grouping sorts 32,768
+    keys, past the 16,384 of the workloads above, so the bitonic sort runs
+    global steps between its LDS chunks; against the oracle."""
+    from numbacs_amd.extraction import ftle_ridges
+
+    nx, ny = 1200, 300
+    x, y = np.linspace(0, 3, nx), np.linspace(-1, 1, ny)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    f = 2 + np.cos(200 * X + 0.3 * np.sin(3 * Y)) + 0.1 * np.cos(7 * Y)
+    ev = np.zeros((nx, ny, 2))
+    ev[..., 0] = 1.0
+    ridges = ftle_ridges(f, ev, x, y, min_ridge_pts=3)
+    ridges_ref = oracle.ridges(f, ev, x, y, percentile_=0, min_ridge_pts=3)
+    assert sum(len(r) for r in ridges_ref) > 16384
+    assert len(ridges) == len(ridges_ref) == 95
+    for a, b in zip(ridges, ridges_ref):
+        assert a.shape == b.shape and np.allclose(a, b, rtol=0, atol=1e-14)
+
+
 def test_ridges_empty_and_degenerate(nb):
     from numbacs_amd.extraction import ftle_ridge_pts, ftle_ridges
 
