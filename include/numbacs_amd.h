@@ -65,7 +65,9 @@ int nbx_flow_create_analytic(int kind, nbx_flow** out);
  * (nt+2, nx+2, ny+2) float64 (get_interp_arrays_2D's output layout).
  * The handle keeps its own interleaved (u, v) copy in HBM.  spherical:
  * 0 planar, 1 lon in [-180,180), 2 lon in [0,360) (degrees, km/h velocities
- * converted with radius r). */
+ * converted with radius r).  extrap (nbx_extrap): 0 zero outside the grid
+ * box, 1 the value plus gradient times offset at the clamped point, 2 the
+ * value at the clamped point. */
 int nbx_flow_create_spline3(const double* grid9_host, const double* Cu, const double* Cv,
                             int spherical, int extrap, double r, nbx_flow** out,
                             nbx_stream stream);
@@ -141,7 +143,9 @@ int nbx_flowmap_compose(const double* flowmaps, int nslots, int first, int nT, d
 
 /* Launch shape of the flow-map kernels for one (flow kind, precision):
  * lanes_per_particle (1, 2 or 4; spline flows only: the lanes of a group
- * split each tricubic gather), threads per block (64, 128 or 256), and
+ * split each tricubic gather; spline flows created with linear
+ * extrapolation always run one lane per particle, in the kernel that
+ * carries the gradient path), threads per block (64, 128 or 256), and
  * persistent_blocks_per_cu (0: one lane group per particle in one launch;
  * n > 0: a persistent grid of n blocks per compute unit pulling particles
  * from a device work queue; grid flow maps and batches only).
