@@ -12,6 +12,12 @@
  *     a call returns once its work is enqueued unless it says otherwise.
  *   - Return value 0 = success, < 0 = error; nbx_last_error() then returns a
  *     thread-local message.
+ *   - The percentile, ridge, label and ordered-ridge entry points share a
+ *     grow-only device scratch buffer per device, held until the process
+ *     ends: their calls must not overlap in time on one device (different
+ *     host threads or streams race on it).  The flow-map, FTLE and
+ *     Cauchy-Green entry points use only their arguments (and, for the
+ *     persistent launch shape, a queue counter allocated per call).
  *
  * The reference's own FFI for this path is numbalsoda's
  *   dop853(funcptr, u0, t_eval, rtol, atol, data=params)
