@@ -59,7 +59,7 @@ def parse():
                          "autotuner (e.g. 4,128,0)")
     ap.add_argument("--workload", default="gridded",
                     choices=["gridded", "double_gyre", "timeseries", "precision", "surrogate",
-                             "composition", "copycal"])
+                             "composition", "next", "copycal"])
     ap.add_argument("--surrogate-samples", type=int, default=4_000_000)
     ap.add_argument("--surrogate-t0s", type=int, default=2)
     return ap.parse_args()
@@ -435,6 +435,134 @@ def bench_composition(args):
             "ftle_vs_direct_frac_within_1e-2": float(errs[:, 2].mean())}
 
 
+def bench_next(args):
+    """The rows next to the hot path, on the gridded-velocity workload's field
+    and particles (676x251, t0 = 72 h, T = -72 h): ordered ridges
+    (ridges.py:720-1054), the aux-grid flow map with its Cauchy-Green eigen
+    stencils (integration.py:249-464, diagnostics.py:113-197), the trilinear
+    flow (flows.py:418-506) and the 'xy' -> 'ij' velocity ingest; GPU times
+    with the oracle's results and times beside each."""
+    import oracle
+    from oracle import ordered as O
+    from numbacs_amd import _lib
+    from numbacs_amd.diagnostics import ftle_eig_2d_device
+    from numbacs_amd.extraction.ridges import ftle_ordered_ridges
+    from numbacs_amd.flows import get_flow_linear_2D, ingest_xy
+    from numbacs_amd.integration import flowmap_aux_grid_2d_device, flowmap_grid_2d_device
+    from numbacs_amd.workloads import particle_axes, synthetic_velocity
+
+    W = gridded_field()
+    _, _, _, U, V = synthetic_velocity()
+    xp, yp = particle_axes()
+    nx, ny = len(xp), len(yp)
+    P = nx * ny
+    dx = float(xp[1] - xp[0])
+    xd, yd = _lib.to_dev(xp), _lib.to_dev(yp)
+    t0, T = 72.0, -72.0
+    steps = max(3, args.steps // 4)
+    # every 40th particle row for the oracle comparisons (the rest masked out)
+    sample = np.ones((nx, ny), bool)
+    sample[::40] = False
+    rows = ~sample[:, 0]
+    S = oracle.Spline(W["grid"], _host(W["Cu"]), _host(W["Cv"]), spherical=1)
+
+    def agree(got, want, sens=None):
+        scale = np.abs(want).max()
+        e = np.abs(got - want).max(-1)
+        r = {"frac_within_1e-6": float(np.mean(e <= 1e-6 * scale)),
+             "max_rel": float(e.max() / scale)}
+        if sens is not None:
+            # a rounding-level difference in an accept/reject decision injects
+            # an error of order rtol that the trajectory then amplifies: the
+            # bound is 1e4 x the response to a 1e-9 relative shift of the start
+            r["within_conditioning_bound"] = bool(np.all(e <= np.maximum(1e-6 * scale,
+                                                                         1e4 * sens)))
+        return r
+
+    def sensitivity(run):
+        base = run(xp, yp)
+        s1 = np.abs(run(xp + 1e-9 * np.ptp(xp), yp) - base).max(-1)
+        s2 = np.abs(run(xp, yp + 1e-9 * np.ptp(yp)) - base).max(-1)
+        return base, np.maximum(s1, s2)
+
+    out = {"workload": "rows next to the hot path on the gridded-velocity field (271x101x96, "
+                       "spherical=1) and particles (676x251), t0=72 h, T=-72 h, FP64; oracle "
+                       "comparisons on every 40th particle row"}
+
+    # ordered ridges on the headline FTLE field (GPU predicate, native host linking)
+    fm = _lib.empty((nx, ny, 2))
+    flowmap_grid_2d_device(W["flow"], t0, T, xd, yd, [-1.0], out=fm)
+    ftle, evec = ftle_eig_2d_device(fm, T, dx, dx)
+    tol = 2 * dx
+    sec, curves = timed(lambda: ftle_ordered_ridges(ftle, evec, xp, yp, tol, percentile=90,
+                                                    min_ridge_pts=5), steps, 1)
+    fh, eh = _host(ftle), _host(evec)
+    a = time.perf_counter()
+    want = O.ordered_ridges(fh, eh, xp, yp, tol, percentile=90, min_ridge_pts=5)
+    cpu_s = time.perf_counter() - a
+    out["ordered_ridges"] = {
+        "what": "ftle_ordered_ridges(ftle, eigvec_max, x, y, dist_tol=0.4, percentile=90, "
+                "min_ridge_pts=5) on the step's FTLE field",
+        "ms": 1e3 * sec, "curves": len(curves), "points": int(sum(len(c) for c in curves)),
+        "oracle_curves": len(want), "oracle_points": int(sum(len(c) for c in want)),
+        "equal_to_oracle": bool(len(curves) == len(want) and
+                                all(np.array_equal(c, w) for c, w in zip(curves, want))),
+        "oracle_python_ms": 1e3 * cpu_s}
+
+    # aux-grid flow map (5 particles per point) + Cauchy-Green eigen from it
+    aux = _lib.empty((nx, ny, 5, 2))
+    ev = _lib.empty((nx, ny, 2))
+    vv = _lib.empty((nx, ny, 2, 2))
+    h_aux = 1e-5
+
+    def aux_step():
+        flowmap_aux_grid_2d_device(W["flow"], t0, T, xd, yd, [-1.0], h=h_aux, out=aux)
+        _lib.check(_lib.load().nbx_C_eig_aux_2d(_lib.ptr(aux), nx, ny, 5, dx, dx, h_aux, 1,
+                                                None, _lib.ptr(ev), _lib.ptr(vv),
+                                                _lib.stream_ptr()))
+
+    sec, _ = timed(aux_step, steps, 1)
+    ah = _host(aux)
+    ref, sens = sensitivity(lambda xx, yy: oracle.flowmap_aux_grid_2d(
+        "spline", np.array([-1.0]), t0, T, xx, yy, h=h_aux, spline=S, mask=sample))
+    out["aux_grid"] = {
+        "what": "flowmap_aux_grid_2D (h=1e-5, eig_main) + C_eig_aux_2D",
+        "ms": 1e3 * sec, "grid_points_per_s": P / sec, "particles_per_s": 5 * P / sec,
+        "centre_vs_flowmap_max_abs": float(np.abs(ah[:, :, 4] - _host(fm)).max()),
+        "vs_oracle": agree(ah[rows].reshape(-1, 2), ref[rows].reshape(-1, 2),
+                           sens[rows].reshape(-1))}
+
+    # trilinear flow over the same velocity samples
+    flin = get_flow_linear_2D(W["grid"], U, V, spherical=1)
+    fml = _lib.empty((nx, ny, 2))
+    nsl = _lib.empty((nx, ny), "i32")
+    sec, _ = timed(lambda: flowmap_grid_2d_device(flin, t0, T, xd, yd, [-1.0], out=fml,
+                                                  nsteps=nsl), steps, 1)
+    spline_steps = _lib.empty((nx, ny), "i32")
+    flowmap_grid_2d_device(W["flow"], t0, T, xd, yd, [-1.0], out=fm, nsteps=spline_steps)
+    L = oracle.Linear(W["grid"], U, V, spherical=1)
+    ref, sens = sensitivity(lambda xx, yy: oracle.flowmap_grid_2d(
+        "linear", np.array([-1.0]), t0, T, xx, yy, spline=L, mask=sample)[0])
+    out["trilinear"] = {
+        "what": "flowmap_grid_2D over get_flow_linear_2D (trilinear, spherical=1)",
+        "ms": 1e3 * sec, "grid_points_per_s": P / sec,
+        "mean_steps_per_particle": float(_host(nsl).mean()),
+        "spline_mean_steps_per_particle": float(_host(spline_steps).mean()),
+        "vs_oracle": agree(_host(fml)[rows], ref[rows], sens[rows])}
+
+    # 'xy' -> 'ij' ingest of the 480-frame field at m/s (x 3.6 -> km/h)
+    _, _, _, U4, _ = synthetic_velocity(480)
+    uxy = _lib.to_dev(np.ascontiguousarray(np.moveaxis(U4, 1, 2)) / 3.6)
+    sec, uij = timed(lambda: ingest_xy(uxy, 3.6), steps, 1)
+    nbytes = 2 * U4.nbytes
+    exact = np.array_equal(_host(uij), np.moveaxis(_host(uxy), 1, 2) * 3.6)
+    out["ingest_xy"] = {
+        "what": "ingest_xy of one (480, 101, 271) component, x3.6",
+        "ms": 1e3 * sec, "hbm_gb_per_s": nbytes / sec / 1e9, "hbm_frac": nbytes / sec / 8e12,
+        "bytes_read_and_written": nbytes, "exact": bool(exact)}
+    return out
+
+
 def bench_precision(args):
     import oracle
     from numbacs_amd import _lib
@@ -597,7 +725,7 @@ def main():
     fn = {"gridded": bench_gridded, "double_gyre": bench_double_gyre,
           "timeseries": bench_timeseries, "precision": bench_precision,
           "surrogate": bench_surrogate, "composition": bench_composition,
-          "copycal": bench_copycal}[args.workload]
+          "next": bench_next, "copycal": bench_copycal}[args.workload]
     print(json.dumps(fn(args)))
 
 

@@ -184,6 +184,10 @@ __global__ __launch_bounds__(256) void k_ridge_predicate(
     const double* __restrict__ ys, int nx, int ny, double dx, double dy, double sdd_thresh,
     const double* __restrict__ fmin_dev, int32_t* __restrict__ flag, double2* __restrict__ pts,
     double* __restrict__ sdd_out) {
+    // the reference's expressions in its source order, uncontracted: ridge
+    // points and second derivatives equal the oracle's bit for bit, so the
+    // ordered-ridge linking (which compares them) takes the same decisions
+#pragma clang fp contract(off)
     const int j = blockIdx.x * 64 + threadIdx.x;
     const int i = blockIdx.y * 4 + threadIdx.y;
     if (i >= nx || j >= ny) return;
