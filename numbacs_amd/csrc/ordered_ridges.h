@@ -289,9 +289,10 @@ inline int64_t total_len(const Linked& L, const std::vector<double>& tags) {
     return s;
 }
 
-// ftle_ordered_ridges' joining loop over the linked ridges.
-inline void join(const Linked& L, double tol, double ang, int64_t min_pts, std::vector<double>& out,
-                 std::vector<int64_t>& offsets) {
+// ftle_ordered_ridges' joining loop over the linked ridges, scanning every
+// remaining endpoint at each query (the restatement's own form).
+inline void join_scan(const Linked& L, double tol, double ang, int64_t min_pts,
+                      std::vector<double>& out, std::vector<int64_t>& offsets) {
     Endpoints R{L.ep, L.eptan};
     const int64_t nridges = static_cast<int64_t>(L.end.size());
     offsets.assign(1, 0);
@@ -337,6 +338,211 @@ inline void join(const Linked& L, double tol, double ang, int64_t min_pts, std::
         if (check_len && total_len(L, tags) < min_pts) continue;
         if (tags.size() == 1) {
             // one piece, in stored order
+            const int64_t r = static_cast<int64_t>(std::llround(std::fabs(tags[0])));
+            for (int64_t q = L.end[r] - L.len[r]; q < L.end[r]; ++q)
+                out.insert(out.end(), {L.pts[2 * q], L.pts[2 * q + 1]});
+            offsets.push_back(static_cast<int64_t>(out.size() / 2));
+        } else {
+            assemble(L, tags, out, offsets);
+        }
+    }
+}
+
+// The same joining loop with the endpoints in a uniform grid of cell size
+// tol.  A query returns the live endpoints within tol (the only ones the
+// scan's argmin can pick: all others are at tol or beyond, rejected ones are
+// set to 10 tol), and trying them in (distance, row) order is the scan's
+// argmin sequence with its first-row tie rule; removals keep the rows'
+// relative order, so original indices order them as the scan's rows.  The
+// cost is O(endpoints x neighbours) instead of O(endpoints^2).
+struct EndpointGrid {
+    const std::vector<double>& e;  // 3 per row: x, y, tag
+    const std::vector<double>& t;  // 2 per row
+    std::vector<char> live;
+    double x0 = 0, y0 = 0, cs = 1;
+    int64_t ncx = 0, ncy = 0, first = 0;
+    std::vector<int64_t> start, ids;  // cells in CSR form, rows ascending per cell
+
+    struct Hit {
+        int64_t k;
+        double d, dx, dy;
+    };
+
+    EndpointGrid(const std::vector<double>& e_, const std::vector<double>& t_) : e(e_), t(t_) {}
+
+    // false when the grid cannot stand in for the scan (non-finite data or a
+    // non-positive tolerance).  Cells are at least tol wide (so a point
+    // within tol of a query lies in the query's cell or a neighbour, one
+    // more for rounding) and doubled until there are at most ~2 per endpoint.
+    bool build(double tol) {
+        const int64_t m = static_cast<int64_t>(e.size() / 3);
+        if (!(tol > 0) || !std::isfinite(tol)) return false;
+        double x1 = 0, y1 = 0;
+        x0 = y0 = 0;
+        for (int64_t k = 0; k < m; ++k) {
+            const double x = e[3 * k], y = e[3 * k + 1];
+            if (!std::isfinite(x) || !std::isfinite(y)) return false;
+            if (k == 0 || x < x0) x0 = x;
+            if (k == 0 || y < y0) y0 = y;
+            if (k == 0 || x > x1) x1 = x;
+            if (k == 0 || y > y1) y1 = y;
+        }
+        cs = tol;
+        double fx = std::floor((x1 - x0) / cs) + 1, fy = std::floor((y1 - y0) / cs) + 1;
+        while (!(fx * fy <= 2.0 * m + 16)) {
+            cs *= 2;
+            fx = std::floor((x1 - x0) / cs) + 1;
+            fy = std::floor((y1 - y0) / cs) + 1;
+        }
+        ncx = static_cast<int64_t>(fx);
+        ncy = static_cast<int64_t>(fy);
+        std::vector<int64_t> cell(static_cast<size_t>(m));
+        start.assign(static_cast<size_t>(ncx * ncy + 1), 0);
+        for (int64_t k = 0; k < m; ++k) {
+            cell[k] = cell_of(e[3 * k], e[3 * k + 1]);
+            ++start[cell[k] + 1];
+        }
+        for (int64_t c = 0; c < ncx * ncy; ++c) start[c + 1] += start[c];
+        ids.assign(static_cast<size_t>(m), 0);
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int64_t k = 0; k < m; ++k) ids[fill[cell[k]]++] = k;
+        live.assign(static_cast<size_t>(m), 1);
+        first = 0;
+        return true;
+    }
+    int64_t cell_of(double x, double y) const {
+        const int64_t a = std::min<int64_t>(
+            ncx - 1, std::max<int64_t>(0, static_cast<int64_t>(std::floor((x - x0) / cs))));
+        const int64_t b = std::min<int64_t>(
+            ncy - 1, std::max<int64_t>(0, static_cast<int64_t>(std::floor((y - y0) / cs))));
+        return a * ncy + b;
+    }
+    bool empty() {
+        const int64_t m = static_cast<int64_t>(live.size());
+        while (first < m && !live[first]) ++first;
+        return first >= m;
+    }
+    // live endpoints strictly within tol of (qx, qy), distances as the scan
+    // computes them; 5x5 cells so that rounding at a cell edge loses none
+    void query(double qx, double qy, double tol, std::vector<Hit>& hits) const {
+        hits.clear();
+        const double fx = std::floor((qx - x0) / cs), fy = std::floor((qy - y0) / cs);
+        if (!std::isfinite(fx) || !std::isfinite(fy)) return;
+        const int64_t ix = static_cast<int64_t>(std::max(-4.0, std::min(fx, double(ncx) + 3)));
+        const int64_t iy = static_cast<int64_t>(std::max(-4.0, std::min(fy, double(ncy) + 3)));
+        for (int64_t a = std::max<int64_t>(0, ix - 2); a <= std::min<int64_t>(ncx - 1, ix + 2); ++a)
+            for (int64_t b = std::max<int64_t>(0, iy - 2); b <= std::min<int64_t>(ncy - 1, iy + 2); ++b) {
+                const int64_t c = a * ncy + b;
+                for (int64_t q = start[c]; q < start[c + 1]; ++q) {
+                    const int64_t k = ids[q];
+                    if (!live[k]) continue;
+                    const double ddx = e[3 * k] - qx, ddy = e[3 * k + 1] - qy;
+                    const double d = std::sqrt(ddx * ddx + ddy * ddy);
+                    if (d < tol) hits.push_back({k, d, ddx, ddy});
+                }
+            }
+    }
+};
+
+inline bool connect_grid(EndpointGrid& R, double& ex, double& ey, double& etag, double& tx,
+                         double& ty, double ang, double tol, double& connect_tag,
+                         std::vector<EndpointGrid::Hit>& hits) {
+    R.query(ex, ey, tol, hits);
+    if (hits.empty()) return false;
+    std::sort(hits.begin(), hits.end(), [](const EndpointGrid::Hit& u, const EndpointGrid::Hit& v) {
+        return u.d < v.d || (u.d == v.d && u.k < v.k);
+    });
+    const double es = sgn(etag);
+    for (const auto& h : hits) {
+        const int64_t k = h.k;
+        const double s = std::sqrt(h.dx * h.dx + h.dy * h.dy);
+        const double ux = h.dx / s, uy = h.dy / s;
+        const double ns = sgn(R.e[3 * k + 2]);
+        const double ntx = R.t[2 * k], nty = R.t[2 * k + 1];
+        if (clamped_acos(-es * (ux * tx + uy * ty)) < ang &&
+            clamped_acos(ns * (ntx * ux + nty * uy)) < ang) {
+            const double tagk = R.e[3 * k + 2];
+            const int64_t p = k + static_cast<int64_t>(sgn(tagk));
+            connect_tag = tagk;
+            ex = R.e[3 * p];
+            ey = R.e[3 * p + 1];
+            etag = R.e[3 * p + 2];
+            tx = R.t[2 * p];
+            ty = R.t[2 * p + 1];
+            R.live[k] = 0;
+            R.live[p] = 0;
+            return true;
+        }
+    }
+    return false;
+}
+
+inline std::vector<double> chain_grid(EndpointGrid& R, const double* cur, const double* curtan,
+                                      double ang, double tol, int64_t nridges,
+                                      std::vector<EndpointGrid::Hit>& hits) {
+    std::vector<double> tags{-(cur[2] + 0.1)};
+    double ex = cur[0], ey = cur[1], etag = cur[2], tx = curtan[0], ty = curtan[1];
+    for (int64_t k = 0; k < nridges; ++k) {
+        double ct;
+        if (!connect_grid(R, ex, ey, etag, tx, ty, ang, tol, ct, hits)) break;
+        tags.push_back(ct);
+    }
+    return tags;
+}
+
+inline void join(const Linked& L, double tol, double ang, int64_t min_pts, std::vector<double>& out,
+                 std::vector<int64_t>& offsets) {
+    EndpointGrid R(L.ep, L.eptan);
+    if (!R.build(tol)) {
+        join_scan(L, tol, ang, min_pts, out, offsets);
+        return;
+    }
+    const int64_t nridges = static_cast<int64_t>(L.end.size());
+    std::vector<EndpointGrid::Hit> hits;
+    offsets.assign(1, 0);
+    while (!R.empty()) {
+        const int64_t i0 = R.first;  // a pair's start row: pairs leave together
+        const double cur[6] = {L.ep[3 * i0],     L.ep[3 * i0 + 1], L.ep[3 * i0 + 2],
+                               L.ep[3 * i0 + 3], L.ep[3 * i0 + 4], L.ep[3 * i0 + 5]};
+        const double ctan[4] = {L.eptan[2 * i0], L.eptan[2 * i0 + 1], L.eptan[2 * i0 + 2],
+                                L.eptan[2 * i0 + 3]};
+        R.live[i0] = 0;
+        R.live[i0 + 1] = 0;
+        double mind[2] = {INFINITY, INFINITY};
+        bool near[2] = {false, false};
+        for (int s = 0; s < 2; ++s) {
+            R.query(cur[3 * s], cur[3 * s + 1], tol, hits);
+            near[s] = !hits.empty();
+            for (const auto& h : hits) mind[s] = std::min(mind[s], h.d);
+        }
+        const int which = (near[0] ? 1 : 0) + (near[1] ? 2 : 0);
+        std::vector<double> tags;
+        bool check_len = true;
+        if (which == 3) {
+            const int s0 = mind[1] < mind[0] ? 1 : 0;
+            const std::vector<double> t0 =
+                chain_grid(R, cur + 3 * s0, ctan + 2 * s0, ang, tol, nridges, hits);
+            const int s1 = s0 + static_cast<int>(sgn(cur[3 * s0 + 2]));
+            const std::vector<double> t1 =
+                chain_grid(R, cur + 3 * s1, ctan + 2 * s1, ang, tol, nridges, hits);
+            if (t0.size() > 1 && t1.size() > 1) {
+                for (auto it = t0.rbegin(); it != t0.rend(); ++it) tags.push_back(-(*it + 0.1));
+                tags.insert(tags.end(), t1.begin() + 1, t1.end());
+            } else if (t0.size() > 1) {
+                tags = t0;
+            } else {
+                tags = t1;
+            }
+            if (t0.size() + t1.size() <= 2) tags.resize(1);
+        } else if (which == 2 || which == 1) {
+            const int s = which == 2 ? 1 : 0;
+            tags = chain_grid(R, cur + 3 * s, ctan + 2 * s, ang, tol, nridges, hits);
+            if (which == 1 && tags.size() == 1) check_len = false;
+        } else {
+            tags = {cur[2]};
+        }
+        if (check_len && total_len(L, tags) < min_pts) continue;
+        if (tags.size() == 1) {
             const int64_t r = static_cast<int64_t>(std::llround(std::fabs(tags[0])));
             for (int64_t q = L.end[r] - L.len[r]; q < L.end[r]; ++q)
                 out.insert(out.end(), {L.pts[2 * q], L.pts[2 * q + 1]});

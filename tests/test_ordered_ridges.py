@@ -160,3 +160,29 @@ def test_native_linking_matches_oracle(case, coords_dg, ftle_data, C_eig_data):
     got = product(f, ev, x, y, dist_tol, **kw)
     assert _same(got, want)
     assert len(want) >= (4 if case == "random" else 1)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("tol", [0.01, 0.07, 0.3, 1.5])
+def test_native_joining_across_tolerances(seed, tol):
+    """Crossing ridges of random direction at tolerances from below the grid
+    step to most of the domain: the library's endpoint joining (a uniform
+    grid of endpoints) takes the restatement's decisions (a scan over all
+    remaining endpoints) exactly."""
+    rng = np.random.default_rng(100 + seed)
+    x, y = _grid(121, 81)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    nr = int(rng.integers(4, 14))
+    ang = rng.uniform(0, np.pi, nr)
+    cx, cy = rng.uniform(0.3, 2.7, nr), rng.uniform(0.2, 1.8, nr)
+    amp = rng.uniform(0.3, 1.0, nr)
+    ridges = np.stack([amp[k] * np.exp(-((np.sin(ang[k]) * (X - cx[k]) -
+                                           np.cos(ang[k]) * (Y - cy[k])) / 0.08) ** 2)
+                       for k in range(nr)])
+    f = ridges.sum(0)
+    top = np.argmax(ridges, axis=0)
+    ev = np.stack([np.sin(ang)[top], -np.cos(ang)[top]], -1)
+    for kw in (dict(), dict(percentile=60, min_ridge_pts=3),
+               dict(ep_tan_ang=math.pi / 2, min_ridge_pts=2)):
+        want = O.ordered_ridges(f, ev, x, y, tol, **kw)
+        assert _same(product(f, ev, x, y, tol, **kw), want)
