@@ -174,16 +174,34 @@ __global__ void k_interleave(const double* __restrict__ u, const double* __restr
 // 'xy' velocity samples (nt, ny, nx) to the 'ij' layout (nt, nx, ny), scaled
 // (the moveaxis + unit conversion of plot_merra_ftle.py:37-41: m/s -> km/h
 // is scale 3.6).  Writes are coalesced along y; the reads stride by nx.
-__global__ void k_ingest_xy(const double* __restrict__ in, int nt, int ny, int nx, double scale,
-                            double* __restrict__ out) {
-    const int64_t n = static_cast<int64_t>(nt) * ny * nx;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int64_t j = k % ny;
-        const int64_t ti = k / ny;
-        const int64_t i = ti % nx;
-        const int64_t t = ti / nx;
-        out[k] = in[(t * ny + j) * nx + i] * scale;
+// One 32x32 tile of one time slice per block, staged in LDS (a padding
+// column keeps the column reads free of bank conflicts): the block reads 32
+// rows of the 'xy' slice along x and writes 32 rows of the 'ij' slice along
+// y, both coalesced.  blockIdx.y runs over (time, y-tile).
+constexpr int kIngestTile = 32;
+constexpr int kIngestRows = 8;  // threads per block: 32 x 8
+
+__global__ __launch_bounds__(kIngestTile * kIngestRows) void k_ingest_xy(
+    const double* __restrict__ in, int nt, int ny, int nx, double scale,
+    double* __restrict__ out) {
+    __shared__ double tile[kIngestTile][kIngestTile + 1];
+    const int nty = (ny + kIngestTile - 1) / kIngestTile;
+    const int64_t t = blockIdx.y / nty;
+    const int by = static_cast<int>(blockIdx.y % nty) * kIngestTile;
+    const int bx = static_cast<int>(blockIdx.x) * kIngestTile;
+    const double* src = in + t * ny * static_cast<int64_t>(nx);
+    double* dst = out + t * nx * static_cast<int64_t>(ny);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+#pragma unroll
+    for (int r = 0; r < kIngestTile; r += kIngestRows) {
+        const int j = by + ty + r, i = bx + tx;
+        if (j < ny && i < nx) tile[ty + r][tx] = src[static_cast<int64_t>(j) * nx + i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kIngestTile; r += kIngestRows) {
+        const int i = bx + ty + r, j = by + tx;
+        if (i < nx && j < ny) dst[static_cast<int64_t>(i) * ny + j] = tile[tx][ty + r] * scale;
     }
 }
 
@@ -509,8 +527,12 @@ int nbx_ingest_xy(const double* U_xy, int nt, int ny, int nx, double scale, doub
     const int64_t n = static_cast<int64_t>(nt) * ny * nx;
     if (n == 0) return 0;
     NBX_REQUIRE(U_xy != nullptr && U_ij != nullptr && U_xy != U_ij, "NULL or aliased array");
-    k_ingest_xy<<<blocks_for(n, 256) < 8192 ? blocks_for(n, 256) : 8192, 256, 0,
-                  as_stream(stream)>>>(U_xy, nt, ny, nx, scale, U_ij);
+    const unsigned tiles_x = static_cast<unsigned>((nx + kIngestTile - 1) / kIngestTile);
+    const int64_t tiles_ty = static_cast<int64_t>(nt) * ((ny + kIngestTile - 1) / kIngestTile);
+    // the dispatch's y extent in work-items (tiles x 8 rows) must fit 32 bits
+    NBX_REQUIRE(tiles_ty * kIngestRows <= 4294967295LL, "too many time slices for one launch");
+    k_ingest_xy<<<dim3(tiles_x, static_cast<unsigned>(tiles_ty)), dim3(kIngestTile, kIngestRows),
+                  0, as_stream(stream)>>>(U_xy, nt, ny, nx, scale, U_ij);
     NBX_LAUNCHED("ingest_xy");
     return 0;
 }

@@ -518,14 +518,17 @@ def test_trilinear_flowmap_vs_oracle(nb):
     assert np.mean(np.abs(f32 - ref).max(-1) <= 1e-3) > 0.95
 
 
-def test_ingest_xy(nb):
+@pytest.mark.parametrize("shape", [(5, 7, 11), (1, 1, 1), (2, 32, 64), (3, 70, 45),
+                                   (4, 33, 97)])
+def test_ingest_xy(nb, shape):
     """'xy' -> 'ij' transpose with the m/s -> km/h factor, exactly as
-    np.moveaxis(u, 1, 2) * 3.6 (plot_merra_ftle.py:37-41)."""
+    np.moveaxis(u, 1, 2) * 3.6 (plot_merra_ftle.py:37-41), on shapes below,
+    at and across the kernel's 32 x 32 tiles."""
     from numbacs_amd import _lib
     from numbacs_amd.flows import ingest_xy
 
     rng = np.random.default_rng(1)
-    u = rng.standard_normal((5, 7, 11))
+    u = rng.standard_normal(shape)
     got = _lib.to_host(ingest_xy(u, 3.6))
     assert np.array_equal(got, np.moveaxis(u, 1, 2) * 3.6)
 
