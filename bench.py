@@ -550,6 +550,22 @@ def bench_next(args):
         "spline_mean_steps_per_particle": float(_host(spline_steps).mean()),
         "vs_oracle": agree(_host(fml)[rows], ref[rows], sens[rows])}
 
+    # natural cubic B-spline prefilter (get_interp_arrays_2D) of one component
+    from numbacs_amd.flows import prefilter3_device
+
+    pre = {}
+    for nt_f in (96, 480):
+        _, _, _, Uf, _ = synthetic_velocity(nt_f)
+        ud = _lib.to_dev(Uf)
+        sec, cd = timed(lambda: prefilter3_device(ud), steps, 1)
+        cref = oracle.prefilter3(Uf)
+        nb = Uf.nbytes + 8 * cref.size
+        pre[f"{nt_f}x271x101"] = {
+            "ms": 1e3 * sec, "hbm_gb_per_s_one_pass": nb / sec / 1e9,
+            "max_rel_vs_oracle": float(np.abs(_host(cd) - cref).max() / np.abs(cref).max())}
+    out["prefilter"] = dict(what="prefilter3 (get_interp_arrays_2D's per-component tricubic "
+                                 "natural B-spline prefilter: scatter + three axis solves)", **pre)
+
     # 'xy' -> 'ij' ingest of the 480-frame field at m/s (x 3.6 -> km/h)
     _, _, _, U4, _ = synthetic_velocity(480)
     uxy = _lib.to_dev(np.ascontiguousarray(np.moveaxis(U4, 1, 2)) / 3.6)
