@@ -24,6 +24,11 @@
 //   * a curve that only its start endpoint could extend, and which found no
 //     partner, is kept whatever its length (the reference's branch for that
 //     case has no min_ridge_pts test).
+//
+// Two ways to the same decisions at lower cost: only the points below the
+// second-derivative threshold are sorted (the walk stops at the first other
+// one), and the joining step finds the endpoints within dist_tol through a
+// uniform grid instead of scanning all remaining endpoints per query.
 #pragma once
 
 #include <algorithm>
