@@ -216,33 +216,10 @@ struct GridSource {
     const double* __restrict__ ys;
     int ny;
     int nx;
-    // The one-solve-per-group launch hands each wavefront a compact tile of
-    // particles (4 x 4 at 4 lanes per particle, 4 x 8 at 2, 8 x 8 at 1)
-    // rather than 64/G consecutive ones along y: a wavefront issues until its
-    // slowest particle is done, and step counts vary smoothly in space, so
-    // a square tile idles fewer lanes (issued solver work x1.20 of the mean
-    // instead of x1.26 on the gridded workload) while its particles still
-    // share stencil rows.  Slots past the grid's edge are empty.
-    template <int G>
-    struct Tile {
-        static constexpr int kPer = 64 / G;          // particles per wavefront
-        static constexpr int kI = G == 1 ? 8 : 4;    // tile rows (x)
-        static constexpr int kJ = kPer / kI;         // tile columns (y)
-    };
-    template <int G>
-    __host__ __device__ int64_t slots() const {
-        using T = Tile<G>;
-        return static_cast<int64_t>((nx + T::kI - 1) / T::kI) * T::kI *
-               ((ny + T::kJ - 1) / T::kJ) * T::kJ;
-    }
-    // particle index of a slot, or -1 for a slot past the edge
-    template <int G>
+    // slots of the grid launch: the tile order of tile_slots/tile_particle
+    __host__ __device__ int64_t slots() const { return tile_slots(nx, ny); }
     __device__ __forceinline__ int64_t particle(int64_t slot) const {
-        using T = Tile<G>;
-        const int64_t w = slot / T::kPer, q = slot % T::kPer;
-        const int64_t ntj = (ny + T::kJ - 1) / T::kJ;
-        const int64_t i = (w / ntj) * T::kI + q / T::kJ, j = (w % ntj) * T::kJ + q % T::kJ;
-        return (i < nx && j < ny) ? i * ny + j : -1;
+        return tile_particle(slot, nx, ny);
     }
     __device__ __forceinline__ bool pos(int64_t p, double& px, double& py) const {
         const int64_t i = p / ny;
@@ -304,7 +281,7 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap(Flow f, 
     int64_t p = tid / G;
     if (p >= P) return;  // whole groups leave together (P counts slots for grids)
     if constexpr (std::is_same<Src, GridSource>::value) {
-        p = src.template particle<G>(p);
+        p = src.particle(p);
         if (p < 0) return;
     }
     const bool lead = (tid & (G - 1)) == 0;
@@ -408,7 +385,7 @@ int launch_solve(const Flow& f, const Src& src, const SolveArgs& a, const Launch
         }
     }
     int64_t slots = a.P;
-    if constexpr (std::is_same<Src, GridSource>::value) slots = src.template slots<G>();
+    if constexpr (std::is_same<Src, GridSource>::value) slots = src.slots();
     k_flowmap<Flow, R, G, Src><<<blocks_for(slots * G, block), block, 0, s>>>(
         f, src, slots, a.s0, a.s1, a.rtol, a.atol, a.mask, a.nmax, a.out, a.status, a.nsteps);
     NBX_LAUNCHED("flowmap");
