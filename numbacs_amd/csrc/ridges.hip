@@ -9,7 +9,8 @@
 // Device pipeline (one host round trip before the end: the ridge-point count,
 // which sizes the grouping stage):
 //   percentile  : two radix selects (8 passes of 8 bits over order-preserving
-//                 64-bit keys) for the order statistics around the virtual
+//                 64-bit keys, each pass's blocks first picking the previous
+//                 pass's digit) for the order statistics around the virtual
 //                 index, then numpy's two-sided lerp -- f_min stays in HBM;
 //   predicate   : one thread per grid point, the reference's central
 //                 differences and Steger sub-pixel test;
@@ -41,11 +42,18 @@ constexpr int kScanTile = kScanThreads * kScanItems;  // 1024 flags per block
 constexpr int kSortThreads = 512;
 constexpr int kSortChunk = 2 * kSortThreads;  // keys per LDS chunk
 
+// Radix-select state.  The digit picked from pass p's histograms is
+// computed by every block of pass p+1 (and by the finishing kernel after the
+// last pass) instead of by a launch of its own, so the histograms rotate
+// over three buffers (pass p adds into hist[p % 3], reads hist[(p-1) % 3]
+// and clears hist[(p+1) % 3], which no block of p reads or writes) and the
+// digit state over two (pass p's blocks read the state of pass p-1's
+// search, block 0 stores the updated one for pass p+1).
 struct SelState {
-    unsigned long long prefix[2];
-    unsigned long long mask[2];
-    long long k[2];
-    unsigned int hist[2][256];
+    unsigned long long prefix[2][2];  // [pass & 1][search]
+    unsigned long long mask[2][2];
+    long long k[2][2];
+    unsigned int hist[3][2][256];     // [pass % 3][search][bin]
     unsigned int nan_count;
     double result;
 };
@@ -63,56 +71,38 @@ __device__ __forceinline__ double key_value(unsigned long long k) {
 __global__ void k_sel_init(SelState* st, long long k_lo, long long k_hi) {
     const int t = threadIdx.x;
     if (t < 256) {
-        st->hist[0][t] = 0;
-        st->hist[1][t] = 0;
+        st->hist[0][0][t] = 0;
+        st->hist[0][1][t] = 0;
     }
     if (t == 0) {
-        st->prefix[0] = st->prefix[1] = 0;
-        st->mask[0] = st->mask[1] = 0;
-        st->k[0] = k_lo;
-        st->k[1] = k_hi;
+        st->prefix[0][0] = st->prefix[0][1] = 0;
+        st->mask[0][0] = st->mask[0][1] = 0;
+        st->k[0][0] = k_lo;
+        st->k[0][1] = k_hi;
         st->nan_count = 0;
         st->result = 0.0;
     }
 }
 
-__global__ __launch_bounds__(256) void k_sel_hist(const double* __restrict__ f, int64_t n,
-                                                  SelState* st, int shift, int count_nan) {
-    __shared__ unsigned int h[2][256];
-    __shared__ unsigned int nans;
-    const int t = threadIdx.x;
-    h[0][t] = 0;
-    h[1][t] = 0;
-    if (t == 0) nans = 0;
-    __syncthreads();
-    const unsigned long long p0 = st->prefix[0], m0 = st->mask[0];
-    const unsigned long long p1 = st->prefix[1], m1 = st->mask[1];
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + t; k < n;
-         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const double v = f[k];
-        if (count_nan && v != v) atomicAdd(&nans, 1u);
-        const unsigned long long key = order_key(v);
-        const unsigned int bin = static_cast<unsigned int>((key >> shift) & 0xffu);
-        if ((key & m0) == p0) atomicAdd(&h[0][bin], 1u);
-        if ((key & m1) == p1) atomicAdd(&h[1][bin], 1u);
+// Digit state of both searches after pass `prev` (digit at `shift`), into
+// LDS: an inclusive scan of the pass's 256-bin histograms, then the bin
+// whose cumulative count first exceeds k.  Called by all 256 threads; ends
+// with a barrier.
+__device__ void sel_advance(const SelState* st, int prev, int shift, long long (*cum)[256],
+                            unsigned long long* sp, unsigned long long* sm, long long* sk) {
+    const int b = threadIdx.x;
+    const int sb = prev & 1, hb = prev % 3;
+    cum[0][b] = st->hist[hb][0][b];
+    cum[1][b] = st->hist[hb][1][b];
+    if (b < 2) {
+        sp[b] = st->prefix[sb][b];
+        sm[b] = st->mask[sb][b];
+        sk[b] = st->k[sb][b];
     }
     __syncthreads();
-    if (h[0][t]) atomicAdd(&st->hist[0][t], h[0][t]);
-    if (h[1][t]) atomicAdd(&st->hist[1][t], h[1][t]);
-    if (t == 0 && nans) atomicAdd(&st->nan_count, nans);
-}
-
-// Pick the digit of both searches: an inclusive scan of the 256-bin
-// histogram in LDS, then the bin whose cumulative count first exceeds k.
-__global__ __launch_bounds__(256) void k_sel_pick(SelState* st, int shift) {
-    __shared__ long long cum[2][256];
-    const int b = threadIdx.x;
     // read both ranks before any thread updates them (the scan's barriers
     // order these reads before the single writer below)
-    const long long kk[2] = {st->k[0], st->k[1]};
-    cum[0][b] = st->hist[0][b];
-    cum[1][b] = st->hist[1][b];
-    __syncthreads();
+    const long long kk[2] = {sk[0], sk[1]};
     for (int off = 1; off < 256; off <<= 1) {
         const long long a0 = b >= off ? cum[0][b - off] : 0;
         const long long a1 = b >= off ? cum[1][b - off] : 0;
@@ -127,19 +117,74 @@ __global__ __launch_bounds__(256) void k_sel_pick(SelState* st, int shift) {
         const long long before = b > 0 ? cum[s][b - 1] : 0;
         // exactly one bin has before <= k < cum (k < total always holds)
         if (before <= k && k < cum[s][b]) {
-            st->k[s] = k - before;
-            st->prefix[s] |= static_cast<unsigned long long>(b) << shift;
-            st->mask[s] |= 0xffull << shift;
+            sk[s] = k - before;
+            sp[s] |= static_cast<unsigned long long>(b) << shift;
+            sm[s] |= 0xffull << shift;
         }
     }
-    st->hist[0][b] = 0;
-    st->hist[1][b] = 0;
+    __syncthreads();
 }
 
-// numpy's _lerp: a + (b-a)*g, or b - (b-a)*(1-g) when g >= 0.5
-__global__ void k_sel_finish(SelState* st, double gamma, double* out) {
-    const double a = key_value(st->prefix[0]);
-    const double b = key_value(st->prefix[1]);
+// Pass `pass` (digit at bits 56 - 8*pass) of both searches.
+__global__ __launch_bounds__(256) void k_sel_hist(const double* __restrict__ f, int64_t n,
+                                                  SelState* st, int pass) {
+    __shared__ unsigned int h[2][256];
+    __shared__ unsigned int nans;
+    __shared__ long long cum[2][256];
+    __shared__ unsigned long long sp[2], sm[2];
+    __shared__ long long sk[2];
+    const int t = threadIdx.x;
+    const int shift = 56 - 8 * pass;
+    h[0][t] = 0;
+    h[1][t] = 0;
+    if (t == 0) nans = 0;
+    if (pass == 0) {
+        if (t < 2) {
+            sp[t] = st->prefix[0][t];
+            sm[t] = st->mask[0][t];
+            sk[t] = st->k[0][t];
+        }
+        __syncthreads();
+    } else {
+        sel_advance(st, pass - 1, shift + 8, cum, sp, sm, sk);
+    }
+    if (blockIdx.x == 0) {
+        if (pass > 0 && t < 2) {
+            st->prefix[pass & 1][t] = sp[t];
+            st->mask[pass & 1][t] = sm[t];
+            st->k[pass & 1][t] = sk[t];
+        }
+        st->hist[(pass + 1) % 3][0][t] = 0;
+        st->hist[(pass + 1) % 3][1][t] = 0;
+    }
+    const unsigned long long p0 = sp[0], m0 = sm[0];
+    const unsigned long long p1 = sp[1], m1 = sm[1];
+    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + t; k < n;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double v = f[k];
+        if (pass == 0 && v != v) atomicAdd(&nans, 1u);
+        const unsigned long long key = order_key(v);
+        const unsigned int bin = static_cast<unsigned int>((key >> shift) & 0xffu);
+        if ((key & m0) == p0) atomicAdd(&h[0][bin], 1u);
+        if ((key & m1) == p1) atomicAdd(&h[1][bin], 1u);
+    }
+    __syncthreads();
+    unsigned int* gh = st->hist[pass % 3][0];
+    if (h[0][t]) atomicAdd(&gh[t], h[0][t]);
+    if (h[1][t]) atomicAdd(&gh[256 + t], h[1][t]);
+    if (t == 0 && nans) atomicAdd(&st->nan_count, nans);
+}
+
+// The last digit (pass 7, bits 0-7), then numpy's _lerp: a + (b-a)*g, or
+// b - (b-a)*(1-g) when g >= 0.5.
+__global__ __launch_bounds__(256) void k_sel_finish(SelState* st, double gamma, double* out) {
+    __shared__ long long cum[2][256];
+    __shared__ unsigned long long sp[2], sm[2];
+    __shared__ long long sk[2];
+    sel_advance(st, 7, 0, cum, sp, sm, sk);
+    if (threadIdx.x != 0) return;
+    const double a = key_value(sp[0]);
+    const double b = key_value(sp[1]);
     const double diff = b - a;
     double r = gamma >= 0.5 ? b - diff * (1 - gamma) : a + diff * gamma;
     if (st->nan_count) r = __longlong_as_double(0x7ff8000000000000ll);
@@ -165,13 +210,10 @@ int percentile_device(const double* f, int64_t n, double q, SelState* st, double
     unsigned nb = blocks_for(n, 256);
     if (nb > 1024) nb = 1024;
     for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 56 - 8 * pass;
-        k_sel_hist<<<nb, 256, 0, s>>>(f, n, st, shift, pass == 0);
+        k_sel_hist<<<nb, 256, 0, s>>>(f, n, st, pass);
         NBX_LAUNCHED("sel_hist");
-        k_sel_pick<<<1, 256, 0, s>>>(st, shift);
-        NBX_LAUNCHED("sel_pick");
     }
-    k_sel_finish<<<1, 1, 0, s>>>(st, gamma, out);
+    k_sel_finish<<<1, 256, 0, s>>>(st, gamma, out);
     NBX_LAUNCHED("sel_finish");
     return 0;
 }
