@@ -6,8 +6,9 @@
 // np.percentile (linear method) and scipy.ndimage.label with a full 3x3
 // structure.
 //
-// Device pipeline (one host round trip before the end: the ridge-point count,
-// which sizes the grouping stage):
+// Device pipeline (with a percentile threshold no host round trip before the
+// end, the grouping stage being sized by a bound on the ridge-point count;
+// without one the count is read back to size it):
 //   percentile  : two radix selects (8 passes of 8 bits over order-preserving
 //                 64-bit keys, each pass's blocks first picking the previous
 //                 pass's digit) for the order statistics around the virtual
@@ -780,17 +781,30 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     k_compact<<<nb, 256, 0, s>>>(flag, pos, dense, n, nullptr, idx);
     NBX_LAUNCHED("compact");
     // The grouping stage works on the ridge points only (typically a few
-    // percent of the grid): read their count, then sort and segment that many.
-    int32_t cnt = 0;
-    NBX_HIP(hipMemcpyAsync(&cnt, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NBX_HIP(hipStreamSynchronize(s));
-    if (cnt == 0) {
+    // percent of the grid), sized by a bound on their count; its kernels read
+    // the exact count on the device.  With a percentile threshold q the bound
+    // needs no read-back: a ridge point has f > f_min >= the order statistic
+    // of rank floor((n-1) q/100), which at most n-1-that rank values exceed,
+    // so the whole pipeline stays queued behind the work before it.  Without
+    // one the count itself is read back.
+    int64_t m = 0;
+    if (percentile != 0) {
+        const double vi = static_cast<double>(n - 1) * (percentile / 100.0);
+        int64_t klo = static_cast<int64_t>(floor(vi));
+        if (vi >= static_cast<double>(n - 1)) klo = n - 1;
+        m = n - 1 - klo;
+    } else {
+        int32_t cnt = 0;
+        NBX_HIP(hipMemcpyAsync(&cnt, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        NBX_HIP(hipStreamSynchronize(s));
+        m = cnt;
+    }
+    if (m == 0) {
         k_offsets_zero<<<1, 1, 0, s>>>(offsets);
         NBX_LAUNCHED("offsets_zero");
         NBX_HIP(hipStreamSynchronize(s));
         return 0;
     }
-    const int64_t m = cnt;
     int64_t Ns = kSortChunk;
     while (Ns < m) Ns <<= 1;
     const unsigned nbm = blocks_for(m, 256);

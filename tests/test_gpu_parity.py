@@ -431,6 +431,23 @@ def test_ridges_vs_oracle_random(nb, percentile):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("percentile", [20, 97.5, 100])
+def test_ridges_percentile_sized_grouping(nb, percentile):
+    """With a percentile threshold the grouping stage is sized by the bound
+    on the ridge-point count (values above the percentile) instead of the
+    count read back: far above the count (20), close to it (97.5) and zero
+    (100, no point can exceed the maximum); against the oracle."""
+    from numbacs_amd.extraction import ftle_ridges
+
+    f, ev, x, y = _field_and_evec(311, 173, 7)
+    ridges = ftle_ridges(f, ev, x, y, percentile=percentile, min_ridge_pts=3)
+    ridges_ref = oracle.ridges(f, ev, x, y, percentile_=percentile, min_ridge_pts=3)
+    assert len(ridges) == len(ridges_ref)
+    assert (len(ridges) == 0) == (percentile == 100)
+    for a, b in zip(ridges, ridges_ref):
+        assert np.array_equal(a, b)
+
+
 def test_ridges_grouping_beyond_one_workgroup(nb):
     """28,120 ridge points on 95 wavy ridge lines: the grouping sorts 32,768
     keys, past the 16,384 of the workloads above, so the bitonic sort runs
