@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void k_ridge_predicate(
     const double* __restrict__ f, const double2* __restrict__ evec, const double* __restrict__ xs,
     const double* __restrict__ ys, int nx, int ny, double dx, double dy, double sdd_thresh,
     const double* __restrict__ fmin_dev, int32_t* __restrict__ flag, double2* __restrict__ pts,
-    double* __restrict__ sdd_out) {
+    double* __restrict__ sdd_out, int32_t* __restrict__ uf_par) {
     // the reference's expressions in its source order, uncontracted: ridge
     // points and second derivatives equal the oracle's bit for bit, so the
     // ordered-ridge linking (which compares them) takes the same decisions
@@ -263,6 +263,8 @@ __global__ __launch_bounds__(256) void k_ridge_predicate(
     }
     flag[k] = hit;
     if (sdd_out != nullptr) sdd_out[k] = sdd;
+    // union-find start (each ridge point its own root), when labelling follows
+    if (uf_par != nullptr) uf_par[k] = hit ? static_cast<int32_t>(k) : -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -500,12 +502,37 @@ __global__ void k_bitonic_global(unsigned long long* keys, int64_t N, int64_t k,
     }
 }
 
+// Two global steps of stage k in one pass, j and h = j/2: thread t owns the
+// four keys lo, lo+h, lo+j, lo+j+h (lo with bits j and h clear), a set that
+// both steps map onto itself, and all four share the direction of bit k.
+__global__ void k_bitonic_global2(unsigned long long* keys, int64_t N, int64_t k, int64_t h) {
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= N / 4) return;
+    const int64_t lo = 4 * h * (t / h) + (t % h);
+    const bool up = ((lo & k) == 0);
+    unsigned long long a0 = keys[lo], a1 = keys[lo + h], a2 = keys[lo + 2 * h],
+                       a3 = keys[lo + 3 * h];
+    cmp_swap(a0, a2, up);
+    cmp_swap(a1, a3, up);
+    cmp_swap(a0, a1, up);
+    cmp_swap(a2, a3, up);
+    keys[lo] = a0;
+    keys[lo + h] = a1;
+    keys[lo + 2 * h] = a2;
+    keys[lo + 3 * h] = a3;
+}
+
 int bitonic_sort(unsigned long long* keys, int64_t N, hipStream_t s) {
     const unsigned chunks = static_cast<unsigned>(N / kSortChunk);
     k_bitonic_chunk_full<<<chunks, kSortThreads, 0, s>>>(keys);
     NBX_LAUNCHED("bitonic_chunk_full");
     for (int64_t k = 2 * kSortChunk; k <= N; k <<= 1) {
-        for (int64_t j = k >> 1; j >= kSortChunk; j >>= 1) {
+        int64_t j = k >> 1;
+        for (; j >= 2 * kSortChunk; j >>= 2) {
+            k_bitonic_global2<<<blocks_for(N / 4, 256), 256, 0, s>>>(keys, N, k, j >> 1);
+            NBX_LAUNCHED("bitonic_global2");
+        }
+        if (j == kSortChunk) {
             k_bitonic_global<<<blocks_for(N / 2, 256), 256, 0, s>>>(keys, N, k, j);
             NBX_LAUNCHED("bitonic_global");
         }
@@ -608,11 +635,24 @@ struct Arena {
 
 size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
+// Read `count` (<= 16) int32 results back to the host: one DMA into a pinned
+// staging buffer per host thread (allocated on the thread's first call and
+// kept, 64 bytes), then a wait on the stream.  A pageable destination would
+// make the runtime stage each copy itself, tens of microseconds apiece.
+int read_counts(const int32_t* dev, int count, int32_t* host, hipStream_t s) {
+    thread_local int32_t* pinned = nullptr;
+    if (pinned == nullptr) NBX_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), 64));
+    NBX_HIP(hipMemcpyAsync(pinned, dev, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    memcpy(host, pinned, sizeof(int32_t) * count);
+    return 0;
+}
+
 // Ridge predicate + optional percentile; leaves flag/pts_dense in the arena.
 int run_predicate(const double* f, const double* evec, const double* x, const double* y, int nx,
                   int ny, double dx, double dy, double sdd_thresh, double percentile,
                   SelState* st, int32_t* flag, double2* pts_dense, hipStream_t s,
-                  double* sdd = nullptr) {
+                  double* sdd = nullptr, int32_t* uf_par = nullptr) {
     const int64_t n = static_cast<int64_t>(nx) * ny;
     const double* fmin_dev = nullptr;
     if (percentile != 0) {
@@ -623,7 +663,7 @@ int run_predicate(const double* f, const double* evec, const double* x, const do
     const dim3 block(64, 4), grid((ny + 63) / 64, (nx + 3) / 4);
     k_ridge_predicate<<<grid, block, 0, s>>>(f, reinterpret_cast<const double2*>(evec), x, y, nx,
                                              ny, dx, dy, sdd_thresh, fmin_dev, flag, pts_dense,
-                                             sdd);
+                                             sdd, uf_par);
     NBX_LAUNCHED("ridge_predicate");
     return 0;
 }
@@ -700,8 +740,7 @@ int nbx_ridge_pts(const double* f, const double* evec, const double* x, int nx, 
                                                  reinterpret_cast<double2*>(pts), nullptr);
     NBX_LAUNCHED("compact");
     int32_t cnt = 0;
-    NBX_HIP(hipMemcpyAsync(&cnt, total, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NBX_HIP(hipStreamSynchronize(s));
+    if (int rc = read_counts(total, 1, &cnt, s)) return rc;
     *npts_host = cnt;
     return 0;
 }
@@ -745,8 +784,8 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
             out_off = A.take<int32_t>(n);
             ridge_idx = A.take<int32_t>(n);
             nseg = A.take<int32_t>(1);
-            total = A.take<int32_t>(1);
-            nridges = A.take<int32_t>(1);
+            total = A.take<int32_t>(2);
+            nridges = total + 1;
         }
     } L;
     Arena measure{nullptr, 0};
@@ -765,12 +804,10 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     unsigned long long* keys = L.keys;
 
     if (int rc = run_predicate(f, evec, x, y, nx, ny, dx, dy, sdd_thresh, percentile, st, flag,
-                               dense, s))
+                               dense, s, nullptr, par))
         return rc;
-    // label components
+    // label components (the predicate started the union-find)
     const unsigned nb = blocks_for(n, 256);
-    k_uf_init<<<nb, 256, 0, s>>>(flag, n, par);
-    NBX_LAUNCHED("uf_init");
     const dim3 block(64, 4), grid((ny + 63) / 64, (nx + 3) / 4);
     k_uf_union<<<grid, block, 0, s>>>(flag, nx, ny, par);
     NBX_LAUNCHED("uf_union");
@@ -795,8 +832,7 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
         m = n - 1 - klo;
     } else {
         int32_t cnt = 0;
-        NBX_HIP(hipMemcpyAsync(&cnt, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        NBX_HIP(hipStreamSynchronize(s));
+        if (int rc = read_counts(count, 1, &cnt, s)) return rc;
         m = cnt;
     }
     if (m == 0) {
@@ -828,12 +864,10 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     NBX_LAUNCHED("group_scatter");
     k_offsets_end<<<1, 1, 0, s>>>(nridges, total, offsets);
     NBX_LAUNCHED("offsets_end");
-    int32_t h[2] = {0, 0};
-    NBX_HIP(hipMemcpyAsync(&h[0], nridges, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NBX_HIP(hipMemcpyAsync(&h[1], total, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NBX_HIP(hipStreamSynchronize(s));
-    *nridges_host = h[0];
-    *npts_host = h[1];
+    int32_t h[2] = {0, 0};  // total, nridges (adjacent in the arena)
+    if (int rc = read_counts(total, 2, h, s)) return rc;
+    *nridges_host = h[1];
+    *npts_host = h[0];
     return 0;
 }
 
@@ -873,8 +907,7 @@ int nbx_label8(const uint8_t* b, int nx, int ny, int32_t* labels, int32_t* nlabe
     k_label_out<<<nb, 256, 0, s>>>(par, rank, n, labels);
     NBX_LAUNCHED("label_out");
     int32_t cnt = 0;
-    NBX_HIP(hipMemcpyAsync(&cnt, total, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NBX_HIP(hipStreamSynchronize(s));
+    if (int rc = read_counts(total, 1, &cnt, s)) return rc;
     *nlabels_host = cnt;
     return 0;
 }
