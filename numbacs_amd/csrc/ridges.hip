@@ -208,7 +208,10 @@ int percentile_device(const double* f, int64_t n, double q, SelState* st, double
     const double gamma = vi - lo;
     k_sel_init<<<1, 256, 0, s>>>(st, klo, khi);
     NBX_LAUNCHED("sel_init");
-    unsigned nb = blocks_for(n, 256);
+    // about 8 keys per thread: every block adds its nonzero bins to the same
+    // 512 global counters, and on the first passes nearly all keys share a
+    // few bins, so fewer blocks serialise fewer atomics on each counter
+    unsigned nb = blocks_for(n, 256 * 8);
     if (nb > 1024) nb = 1024;
     for (int pass = 0; pass < 8; ++pass) {
         k_sel_hist<<<nb, 256, 0, s>>>(f, n, st, pass);
