@@ -271,38 +271,51 @@ __global__ __launch_bounds__(256) void k_ridge_predicate(
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of int32 values (sums must fit in int32).
+// Exclusive scan of int32 values, or of int32 pairs scanned together (sums
+// must fit in int32).
 
-__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* sh, int32_t* total) {
+struct I2 {
+    int32_t a, b;
+};
+__device__ __forceinline__ I2 operator+(I2 x, I2 y) { return I2{x.a + y.a, x.b + y.b}; }
+__device__ __forceinline__ I2& operator+=(I2& x, I2 y) {
+    x.a += y.a;
+    x.b += y.b;
+    return x;
+}
+
+template <class T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* sh, T* total) {
     const int t = threadIdx.x;
     sh[t] = v;
     __syncthreads();
     for (int off = 1; off < kScanThreads; off <<= 1) {
-        const int32_t add = t >= off ? sh[t - off] : 0;
+        const T add = t >= off ? sh[t - off] : T{};
         __syncthreads();
         sh[t] += add;
         __syncthreads();
     }
-    const int32_t incl = sh[t];
+    const T ex = t > 0 ? sh[t - 1] : T{};
     *total = sh[kScanThreads - 1];
     __syncthreads();
-    return incl - v;
+    return ex;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const int32_t* __restrict__ in,
-                                                             int64_t n, int32_t* __restrict__ out,
-                                                             int32_t* __restrict__ tile_sums) {
-    __shared__ int32_t sh[kScanThreads];
+template <class T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const T* __restrict__ in, int64_t n,
+                                                             T* __restrict__ out,
+                                                             T* __restrict__ tile_sums) {
+    __shared__ T sh[kScanThreads];
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
-    int32_t v[kScanItems];
-    int32_t sum = 0;
+    T v[kScanItems];
+    T sum{};
 #pragma unroll
     for (int q = 0; q < kScanItems; ++q) {
-        v[q] = base + q < n ? in[base + q] : 0;
+        v[q] = base + q < n ? in[base + q] : T{};
         sum += v[q];
     }
-    int32_t total;
-    int32_t run = block_exclusive_scan(sum, sh, &total);
+    T total;
+    T run = block_exclusive_scan(sum, sh, &total);
 #pragma unroll
     for (int q = 0; q < kScanItems; ++q) {
         if (base + q < n) out[base + q] = run;
@@ -312,33 +325,38 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const int32_t* __re
 }
 
 // single block: exclusive scan of the tile sums in place; grand total to *total
-__global__ __launch_bounds__(kScanThreads) void k_scan_sums(int32_t* __restrict__ sums,
-                                                            int64_t m, int32_t* __restrict__ total) {
-    __shared__ int32_t sh[kScanThreads];
-    int32_t carry = 0;
+template <class T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(T* __restrict__ sums, int64_t m,
+                                                            T* __restrict__ total) {
+    __shared__ T sh[kScanThreads];
+    T carry{};
     for (int64_t base = 0; base < m; base += kScanThreads) {
         const int64_t k = base + threadIdx.x;
-        const int32_t v = k < m ? sums[k] : 0;
-        int32_t chunk;
-        const int32_t ex = block_exclusive_scan(v, sh, &chunk);
+        const T v = k < m ? sums[k] : T{};
+        T chunk;
+        const T ex = block_exclusive_scan(v, sh, &chunk);
         if (k < m) sums[k] = carry + ex;
         carry += chunk;
     }
     if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ __launch_bounds__(256) void k_scan_add(int32_t* __restrict__ out, int64_t n,
-                                                  const int32_t* __restrict__ tile_sums) {
+template <class T>
+__global__ __launch_bounds__(256) void k_scan_add(T* __restrict__ out, int64_t n,
+                                                  const T* __restrict__ tile_sums) {
     const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (k < n) out[k] += tile_sums[k / kScanTile];
 }
 
-__global__ void k_set_zero(int32_t* p) { *p = 0; }
+template <class T>
+__global__ void k_set_zero(T* p) {
+    *p = T{};
+}
 
 // out = exclusive scan of in; *total (device) = sum.  tmp holds
-// ceil(n / 1024) ints.
-int exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* tmp, int32_t* total,
-                   hipStream_t s) {
+// ceil(n / 1024) values of T.
+template <class T>
+int exclusive_scan(const T* in, int64_t n, T* out, T* tmp, T* total, hipStream_t s) {
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
     if (n == 0) {
         k_set_zero<<<1, 1, 0, s>>>(total);
@@ -586,24 +604,21 @@ __global__ void k_seg_first(const int32_t* __restrict__ seg_start,
 
 // kept ridge sizes: len or 0
 __global__ void k_seg_keep(const int32_t* __restrict__ seg_first, const int32_t* __restrict__ nseg_dev,
-                           int64_t cap, int min_pts, int32_t* __restrict__ kept_len,
-                           int32_t* __restrict__ kept_flag) {
+                           int64_t cap, int min_pts, I2* __restrict__ kept) {
     const int64_t s = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (s >= cap) return;
     int32_t len = 0;
     if (s < *nseg_dev) len = seg_first[s + 1] - seg_first[s];
     const bool keep = len >= min_pts && len > 0;
-    kept_len[s] = keep ? len : 0;
-    kept_flag[s] = keep ? 1 : 0;
+    kept[s] = I2{keep ? len : 0, keep ? 1 : 0};
 }
 
 __global__ void k_group_scatter(const unsigned long long* __restrict__ keys,
                                 const int32_t* __restrict__ seg_start,
                                 const int32_t* __restrict__ seg_ex,
                                 const int32_t* __restrict__ seg_first,
-                                const int32_t* __restrict__ kept_flag,
-                                const int32_t* __restrict__ out_off,
-                                const int32_t* __restrict__ ridge_idx,
+                                const I2* __restrict__ kept,
+                                const I2* __restrict__ out_off_idx,
                                 const double2* __restrict__ pts_dense,
                                 const int32_t* __restrict__ count_dev, int64_t cap,
                                 double2* __restrict__ pts_out, int64_t* __restrict__ offsets) {
@@ -611,11 +626,12 @@ __global__ void k_group_scatter(const unsigned long long* __restrict__ keys,
     if (t >= cap || t >= *count_dev) return;
     // segment of t: inclusive count of starts up to t, minus one
     const int32_t s = seg_ex[t] + seg_start[t] - 1;
-    if (!kept_flag[s]) return;
+    if (!kept[s].b) return;
+    const I2 o = out_off_idx[s];  // output offset, ridge number
     const int32_t k = static_cast<int32_t>(keys[t] & 0xffffffffull);
-    const int32_t r = out_off[s] + (static_cast<int32_t>(t) - seg_first[s]);
+    const int32_t r = o.a + (static_cast<int32_t>(t) - seg_first[s]);
     pts_out[r] = pts_dense[k];
-    if (t == seg_first[s]) offsets[ridge_idx[s]] = out_off[s];
+    if (t == seg_first[s]) offsets[o.b] = o.a;
 }
 
 __global__ void k_offsets_end(const int32_t* __restrict__ nridges_dev,
@@ -765,8 +781,9 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     while (N < n) N <<= 1;
     struct Layout {
         SelState* st;
-        int32_t *flag, *pos, *tmp, *count, *par, *idx, *seg_start, *seg_ex, *seg_first, *kept_len,
-            *kept_flag, *out_off, *ridge_idx, *nseg, *total, *nridges;
+        int32_t *flag, *pos, *tmp, *count, *par, *idx, *seg_start, *seg_ex, *seg_first, *nseg,
+            *total, *nridges;
+        I2 *kept, *out_off_idx;
         double2* dense;
         unsigned long long* keys;
         void carve(Arena& A, int64_t n, int64_t tiles, int64_t N) {
@@ -774,7 +791,7 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
             flag = A.take<int32_t>(n);
             pos = A.take<int32_t>(n);
             dense = A.take<double2>(n);
-            tmp = A.take<int32_t>(tiles + 1);
+            tmp = A.take<int32_t>(2 * (tiles + 1));  // also the pair scan's tile sums
             count = A.take<int32_t>(1);
             par = A.take<int32_t>(n);
             idx = A.take<int32_t>(n);
@@ -782,10 +799,8 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
             seg_start = A.take<int32_t>(n);
             seg_ex = A.take<int32_t>(n);
             seg_first = A.take<int32_t>(n + 1);
-            kept_len = A.take<int32_t>(n);
-            kept_flag = A.take<int32_t>(n);
-            out_off = A.take<int32_t>(n);
-            ridge_idx = A.take<int32_t>(n);
+            kept = A.take<I2>(n);         // (kept length, kept flag) per segment
+            out_off_idx = A.take<I2>(n);  // their exclusive scans
             nseg = A.take<int32_t>(1);
             total = A.take<int32_t>(2);
             nridges = total + 1;
@@ -800,8 +815,7 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     SelState* st = L.st;
     int32_t *flag = L.flag, *pos = L.pos, *tmp = L.tmp, *count = L.count, *par = L.par,
             *idx = L.idx, *seg_start = L.seg_start, *seg_ex = L.seg_ex,
-            *seg_first = L.seg_first, *kept_len = L.kept_len, *kept_flag = L.kept_flag,
-            *out_off = L.out_off, *ridge_idx = L.ridge_idx, *nseg = L.nseg, *total = L.total,
+            *seg_first = L.seg_first, *nseg = L.nseg, *total = L.total,
             *nridges = L.nridges;
     double2* dense = L.dense;
     unsigned long long* keys = L.keys;
@@ -857,12 +871,14 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     if (int rc = exclusive_scan(seg_start, m, seg_ex, tmp, nseg, s)) return rc;
     k_seg_first<<<nbm, 256, 0, s>>>(seg_start, seg_ex, count, nseg, m, seg_first);
     NBX_LAUNCHED("seg_first");
-    k_seg_keep<<<nbm, 256, 0, s>>>(seg_first, nseg, m, min_ridge_pts, kept_len, kept_flag);
+    k_seg_keep<<<nbm, 256, 0, s>>>(seg_first, nseg, m, min_ridge_pts, L.kept);
     NBX_LAUNCHED("seg_keep");
-    if (int rc = exclusive_scan(kept_len, m, out_off, tmp, total, s)) return rc;
-    if (int rc = exclusive_scan(kept_flag, m, ridge_idx, tmp, nridges, s)) return rc;
-    k_group_scatter<<<nbm, 256, 0, s>>>(keys, seg_start, seg_ex, seg_first, kept_flag, out_off,
-                                        ridge_idx, dense, count, m,
+    // total and nridges are adjacent: the pair scan's total lands in both
+    if (int rc = exclusive_scan(L.kept, m, L.out_off_idx, reinterpret_cast<I2*>(tmp),
+                                reinterpret_cast<I2*>(total), s))
+        return rc;
+    k_group_scatter<<<nbm, 256, 0, s>>>(keys, seg_start, seg_ex, seg_first, L.kept,
+                                        L.out_off_idx, dense, count, m,
                                         reinterpret_cast<double2*>(pts), offsets);
     NBX_LAUNCHED("group_scatter");
     k_offsets_end<<<1, 1, 0, s>>>(nridges, total, offsets);
