@@ -372,14 +372,34 @@ int exclusive_scan(const T* in, int64_t n, T* out, T* tmp, T* total, hipStream_t
     return 0;
 }
 
+// The first two phases only: out holds each value's exclusive scan within
+// its 1024-value tile and tile_off the tiles' offsets, so a consumer reads
+// out[k] + tile_off[k / kScanTile] and no third pass is launched.
+template <class T>
+int exclusive_scan_tiled(const T* in, int64_t n, T* out, T* tile_off, T* total, hipStream_t s) {
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (n == 0) {
+        k_set_zero<<<1, 1, 0, s>>>(total);
+        NBX_LAUNCHED("set_zero");
+        return 0;
+    }
+    k_scan_tiles<<<static_cast<unsigned>(tiles), kScanThreads, 0, s>>>(in, n, out, tile_off);
+    NBX_LAUNCHED("scan_tiles");
+    k_scan_sums<<<1, kScanThreads, 0, s>>>(tile_off, tiles, total);
+    NBX_LAUNCHED("scan_sums");
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Compaction of flagged points in raveled order.
+// pos is a complete exclusive scan, or with tile_off the tiled form.
 __global__ void k_compact(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
                           const double2* __restrict__ pts_dense, int64_t n,
-                          double2* __restrict__ pts_out, int32_t* __restrict__ idx_out) {
+                          double2* __restrict__ pts_out, int32_t* __restrict__ idx_out,
+                          const int32_t* __restrict__ tile_off) {
     const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (k >= n || !flag[k]) return;
-    const int32_t p = pos[k];
+    const int32_t p = pos[k] + (tile_off ? tile_off[k / kScanTile] : 0);
     if (pts_out) pts_out[p] = pts_dense[k];
     if (idx_out) idx_out[p] = static_cast<int32_t>(k);
 }
@@ -593,12 +613,16 @@ __global__ void k_seg_start(const unsigned long long* __restrict__ keys,
 }
 
 // seg_first[s] = position of the first key of segment s; seg_first[nseg] = cnt
+// seg_ex/seg_tile: the tiled exclusive scan of seg_start
 __global__ void k_seg_first(const int32_t* __restrict__ seg_start,
-                            const int32_t* __restrict__ seg_ex, const int32_t* __restrict__ count_dev,
+                            const int32_t* __restrict__ seg_ex,
+                            const int32_t* __restrict__ seg_tile,
+                            const int32_t* __restrict__ count_dev,
                             const int32_t* __restrict__ nseg_dev, int64_t cap,
                             int32_t* __restrict__ seg_first) {
     const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (t < cap && t < *count_dev && seg_start[t]) seg_first[seg_ex[t]] = static_cast<int32_t>(t);
+    if (t < cap && t < *count_dev && seg_start[t])
+        seg_first[seg_ex[t] + seg_tile[t / kScanTile]] = static_cast<int32_t>(t);
     if (t == 0) seg_first[*nseg_dev] = *count_dev;
 }
 
@@ -616,27 +640,29 @@ __global__ void k_seg_keep(const int32_t* __restrict__ seg_first, const int32_t*
 __global__ void k_group_scatter(const unsigned long long* __restrict__ keys,
                                 const int32_t* __restrict__ seg_start,
                                 const int32_t* __restrict__ seg_ex,
+                                const int32_t* __restrict__ seg_tile,
                                 const int32_t* __restrict__ seg_first,
                                 const I2* __restrict__ kept,
                                 const I2* __restrict__ out_off_idx,
+                                const I2* __restrict__ out_tile,
+                                const I2* __restrict__ totals,
                                 const double2* __restrict__ pts_dense,
                                 const int32_t* __restrict__ count_dev, int64_t cap,
                                 double2* __restrict__ pts_out, int64_t* __restrict__ offsets) {
     const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t == 0) {
+        const I2 tot = *totals;  // (points kept, ridges kept)
+        offsets[tot.b] = tot.a;
+    }
     if (t >= cap || t >= *count_dev) return;
     // segment of t: inclusive count of starts up to t, minus one
-    const int32_t s = seg_ex[t] + seg_start[t] - 1;
+    const int32_t s = seg_ex[t] + seg_tile[t / kScanTile] + seg_start[t] - 1;
     if (!kept[s].b) return;
-    const I2 o = out_off_idx[s];  // output offset, ridge number
+    const I2 o = out_off_idx[s] + out_tile[s / kScanTile];  // output offset, ridge number
     const int32_t k = static_cast<int32_t>(keys[t] & 0xffffffffull);
     const int32_t r = o.a + (static_cast<int32_t>(t) - seg_first[s]);
     pts_out[r] = pts_dense[k];
     if (t == seg_first[s]) offsets[o.b] = o.a;
-}
-
-__global__ void k_offsets_end(const int32_t* __restrict__ nridges_dev,
-                              const int32_t* __restrict__ total_dev, int64_t* __restrict__ offsets) {
-    offsets[*nridges_dev] = *total_dev;
 }
 
 // Bump allocator over one workspace block.
@@ -756,7 +782,7 @@ int nbx_ridge_pts(const double* f, const double* evec, const double* x, int nx, 
         return rc;
     if (int rc = exclusive_scan(flag, n, pos, tmp, total, s)) return rc;
     k_compact<<<blocks_for(n, 256), 256, 0, s>>>(flag, pos, dense, n,
-                                                 reinterpret_cast<double2*>(pts), nullptr);
+                                                 reinterpret_cast<double2*>(pts), nullptr, nullptr);
     NBX_LAUNCHED("compact");
     int32_t cnt = 0;
     if (int rc = read_counts(total, 1, &cnt, s)) return rc;
@@ -782,8 +808,8 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     struct Layout {
         SelState* st;
         int32_t *flag, *pos, *tmp, *count, *par, *idx, *seg_start, *seg_ex, *seg_first, *nseg,
-            *total, *nridges;
-        I2 *kept, *out_off_idx;
+            *total, *nridges;  // nridges = total + 1: the pair scan's (points, ridges) total
+        I2 *kept, *out_off_idx, *tmp2;
         double2* dense;
         unsigned long long* keys;
         void carve(Arena& A, int64_t n, int64_t tiles, int64_t N) {
@@ -791,7 +817,8 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
             flag = A.take<int32_t>(n);
             pos = A.take<int32_t>(n);
             dense = A.take<double2>(n);
-            tmp = A.take<int32_t>(2 * (tiles + 1));  // also the pair scan's tile sums
+            tmp = A.take<int32_t>(tiles + 1);   // tile offsets: compaction, then segments
+            tmp2 = A.take<I2>(tiles + 1);       // tile offsets of the pair scan
             count = A.take<int32_t>(1);
             par = A.take<int32_t>(n);
             idx = A.take<int32_t>(n);
@@ -815,8 +842,7 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     SelState* st = L.st;
     int32_t *flag = L.flag, *pos = L.pos, *tmp = L.tmp, *count = L.count, *par = L.par,
             *idx = L.idx, *seg_start = L.seg_start, *seg_ex = L.seg_ex,
-            *seg_first = L.seg_first, *nseg = L.nseg, *total = L.total,
-            *nridges = L.nridges;
+            *seg_first = L.seg_first, *nseg = L.nseg, *total = L.total;
     double2* dense = L.dense;
     unsigned long long* keys = L.keys;
 
@@ -831,8 +857,8 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     k_uf_flatten<<<nb, 256, 0, s>>>(flag, n, par);
     NBX_LAUNCHED("uf_flatten");
     // compact ridge points (raveled order)
-    if (int rc = exclusive_scan(flag, n, pos, tmp, count, s)) return rc;
-    k_compact<<<nb, 256, 0, s>>>(flag, pos, dense, n, nullptr, idx);
+    if (int rc = exclusive_scan_tiled(flag, n, pos, tmp, count, s)) return rc;
+    k_compact<<<nb, 256, 0, s>>>(flag, pos, dense, n, nullptr, idx, tmp);
     NBX_LAUNCHED("compact");
     // The grouping stage works on the ridge points only (typically a few
     // percent of the grid), sized by a bound on their count; its kernels read
@@ -868,21 +894,20 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     // segments = components
     k_seg_start<<<nbm, 256, 0, s>>>(keys, count, m, seg_start);
     NBX_LAUNCHED("seg_start");
-    if (int rc = exclusive_scan(seg_start, m, seg_ex, tmp, nseg, s)) return rc;
-    k_seg_first<<<nbm, 256, 0, s>>>(seg_start, seg_ex, count, nseg, m, seg_first);
+    if (int rc = exclusive_scan_tiled(seg_start, m, seg_ex, tmp, nseg, s)) return rc;
+    k_seg_first<<<nbm, 256, 0, s>>>(seg_start, seg_ex, tmp, count, nseg, m, seg_first);
     NBX_LAUNCHED("seg_first");
     k_seg_keep<<<nbm, 256, 0, s>>>(seg_first, nseg, m, min_ridge_pts, L.kept);
     NBX_LAUNCHED("seg_keep");
     // total and nridges are adjacent: the pair scan's total lands in both
-    if (int rc = exclusive_scan(L.kept, m, L.out_off_idx, reinterpret_cast<I2*>(tmp),
-                                reinterpret_cast<I2*>(total), s))
-        return rc;
-    k_group_scatter<<<nbm, 256, 0, s>>>(keys, seg_start, seg_ex, seg_first, L.kept,
-                                        L.out_off_idx, dense, count, m,
+    I2* totals = reinterpret_cast<I2*>(total);
+    if (int rc = exclusive_scan_tiled(L.kept, m, L.out_off_idx, L.tmp2, totals, s)) return rc;
+    // the segment scan's tile offsets (tmp) are still intact: the pair scan
+    // used tmp2; the scatter also writes offsets[nridges] = total
+    k_group_scatter<<<nbm, 256, 0, s>>>(keys, seg_start, seg_ex, tmp, seg_first, L.kept,
+                                        L.out_off_idx, L.tmp2, totals, dense, count, m,
                                         reinterpret_cast<double2*>(pts), offsets);
     NBX_LAUNCHED("group_scatter");
-    k_offsets_end<<<1, 1, 0, s>>>(nridges, total, offsets);
-    NBX_LAUNCHED("offsets_end");
     int32_t h[2] = {0, 0};  // total, nridges (adjacent in the arena)
     if (int rc = read_counts(total, 2, h, s)) return rc;
     *nridges_host = h[1];
