@@ -15,8 +15,9 @@
 //                 index, then numpy's two-sided lerp -- f_min stays in HBM;
 //   predicate   : one thread per grid point, the reference's central
 //                 differences and Steger sub-pixel test;
-//   compaction  : three-phase exclusive scan over the ridge flags, so points
-//                 come out in raveled (i*ny+j) order;
+//   compaction  : exclusive scan over the ridge flags (tile scans, then the
+//                 tile offsets, which consumers of the grouping path add
+//                 themselves), so points come out in raveled (i*ny+j) order;
 //   labelling   : lock-free union-find (atomicMin linking of the larger root
 //                 under the smaller) over the 4 raster-earlier neighbours,
 //                 then path flattening; every root is its component's

@@ -448,10 +448,13 @@ def test_ridges_percentile_sized_grouping(nb, percentile):
         assert np.array_equal(a, b)
 
 
-def test_ridges_grouping_beyond_one_workgroup(nb):
+@pytest.mark.parametrize("percentile", [0, 50])
+def test_ridges_grouping_beyond_one_workgroup(nb, percentile):
     """28,120 ridge points on 95 wavy ridge lines: the grouping sorts 32,768
-    keys, past the 16,384 of the workloads above, so the bitonic sort runs
-    global steps between its LDS chunks; against the oracle."""
+    keys (the count read back, percentile 0) or 262,144 (sized by the
+    median's bound of 180,000, mostly padding), past the 16,384 of the
+    workloads above, so the bitonic sort runs global steps between its LDS
+    chunks; against the oracle."""
     from numbacs_amd.extraction import ftle_ridges
 
     nx, ny = 1200, 300
@@ -460,8 +463,8 @@ def test_ridges_grouping_beyond_one_workgroup(nb):
     f = 2 + np.cos(200 * X + 0.3 * np.sin(3 * Y)) + 0.1 * np.cos(7 * Y)
     ev = np.zeros((nx, ny, 2))
     ev[..., 0] = 1.0
-    ridges = ftle_ridges(f, ev, x, y, min_ridge_pts=3)
-    ridges_ref = oracle.ridges(f, ev, x, y, percentile_=0, min_ridge_pts=3)
+    ridges = ftle_ridges(f, ev, x, y, percentile=percentile, min_ridge_pts=3)
+    ridges_ref = oracle.ridges(f, ev, x, y, percentile_=percentile, min_ridge_pts=3)
     assert sum(len(r) for r in ridges_ref) > 16384
     assert len(ridges) == len(ridges_ref) == 95
     for a, b in zip(ridges, ridges_ref):
