@@ -10,7 +10,10 @@
 #include <cmath>
 #include <cstring>
 #include <algorithm>
+#include <map>
+#include <memory>
 #include <mutex>
+#include <tuple>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -25,45 +28,48 @@ namespace nbx {
 
 namespace {
 thread_local std::string g_last_error;
-std::mutex g_ws_mutex;
-struct WsSlot {
+struct WsEntry {
+    std::mutex m;  // held by the Workspace that uses the buffer
     void* ptr = nullptr;
     size_t bytes = 0;
 };
-std::vector<std::vector<WsSlot>> g_ws;  // [device][slot]
+std::mutex g_ws_mutex;  // guards the map, not the buffers
+std::map<std::tuple<int, hipStream_t, int>, std::unique_ptr<WsEntry>> g_ws;
 }  // namespace
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-void* workspace(int slot, size_t bytes, int* err) {
-    *err = 0;
+Workspace::Workspace(hipStream_t stream, int slot, size_t bytes) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) {
         set_error("hipGetDevice failed");
-        *err = -1;
-        return nullptr;
+        err_ = -1;
+        return;
     }
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    if (static_cast<int>(g_ws.size()) <= dev) g_ws.resize(dev + 1);
-    auto& slots = g_ws[dev];
-    if (static_cast<int>(slots.size()) <= slot) slots.resize(slot + 1);
-    WsSlot& s = slots[slot];
-    if (s.bytes < bytes) {
-        if (s.ptr) {
-            // hipFree synchronises the device, so no queued kernel still uses it
-            (void)hipFree(s.ptr);
-            s.ptr = nullptr;
-            s.bytes = 0;
+    WsEntry* e = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_ws_mutex);
+        auto& slot_entry = g_ws[std::make_tuple(dev, stream, slot)];
+        if (!slot_entry) slot_entry = std::make_unique<WsEntry>();
+        e = slot_entry.get();
+    }
+    lock_ = std::unique_lock<std::mutex>(e->m);
+    if (e->bytes < bytes) {
+        if (e->ptr) {
+            // hipFree waits for the device work still using the old buffer
+            (void)hipFree(e->ptr);
+            e->ptr = nullptr;
+            e->bytes = 0;
         }
-        size_t want = bytes < 4096 ? 4096 : bytes;
-        if (hipMalloc(&s.ptr, want) != hipSuccess) {
+        const size_t want = bytes < 4096 ? 4096 : bytes;
+        if (hipMalloc(&e->ptr, want) != hipSuccess) {
             set_error("workspace hipMalloc failed");
-            *err = -1;
-            return nullptr;
+            err_ = -1;
+            return;
         }
-        s.bytes = want;
+        e->bytes = want;
     }
-    return s.ptr;
+    ptr_ = static_cast<char*>(e->ptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -741,105 +741,68 @@ int check_grid(int nx, int ny) {
     return 0;
 }
 
-}  // namespace
-}  // namespace nbx
+// Scratch layout of the grouped-ridge pipeline for n grid points.
+struct RidgeLayout {
+    SelState* st;
+    // total[0..1] = the pair scan's (points, ridges) total, total[2] = count,
+    // the number of ridge points: one read-back fetches all three
+    int32_t *flag, *pos, *tmp, *count, *par, *idx, *seg_start, *seg_ex, *seg_first, *nseg,
+        *total;
+    I2 *kept, *out_off_idx, *tmp2;
+    double2* dense;
+    unsigned long long* keys;
+    void carve(Arena& A, int64_t n) {
+        const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+        int64_t N = kSortChunk;
+        while (N < n) N <<= 1;
+        st = A.take<SelState>(1);
+        flag = A.take<int32_t>(n);
+        pos = A.take<int32_t>(n);
+        dense = A.take<double2>(n);
+        tmp = A.take<int32_t>(tiles + 1);   // tile offsets: compaction, then segments
+        tmp2 = A.take<I2>(tiles + 1);       // tile offsets of the pair scan
+        par = A.take<int32_t>(n);
+        idx = A.take<int32_t>(n);
+        keys = A.take<unsigned long long>(N);
+        seg_start = A.take<int32_t>(n);
+        seg_ex = A.take<int32_t>(n);
+        seg_first = A.take<int32_t>(n + 1);
+        kept = A.take<I2>(n);         // (kept length, kept flag) per segment
+        out_off_idx = A.take<I2>(n);  // their exclusive scans
+        nseg = A.take<int32_t>(1);
+        total = A.take<int32_t>(3);
+        count = total + 2;
+    }
+    static size_t bytes(int64_t n) {
+        RidgeLayout L;
+        Arena measure{nullptr, 0};
+        L.carve(measure, n);
+        return measure.used;
+    }
+};
 
-using namespace nbx;
-
-extern "C" {
-
-int nbx_percentile(const double* f, int64_t n, double q, double* result, nbx_stream stream) {
-    NBX_REQUIRE(f != nullptr && result != nullptr, "NULL array");
-    int err = 0;
-    auto* st = static_cast<SelState*>(workspace(2, sizeof(SelState), &err));
-    if (err) return err;
-    return percentile_device(f, n, q, st, result, as_stream(stream));
-}
-
-int nbx_ridge_pts(const double* f, const double* evec, const double* x, int nx, const double* y,
-                  int ny, double dx, double dy, double sdd_thresh, double percentile, double* pts,
-                  int64_t* npts_host, nbx_stream stream) {
-    if (int rc = check_grid(nx, ny)) return rc;
-    NBX_REQUIRE(npts_host != nullptr, "npts_host is NULL");
-    const int64_t n = static_cast<int64_t>(nx) * ny;
-    *npts_host = 0;
-    if (n == 0) return 0;
-    NBX_REQUIRE(f && evec && x && y && pts, "NULL array");
-    hipStream_t s = as_stream(stream);
-    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    const size_t bytes = align256(sizeof(SelState)) + align256(4 * n) * 2 + align256(16 * n) +
-                         align256(4 * (tiles + 1)) + align256(4);
-    int err = 0;
-    Arena A{static_cast<char*>(workspace(1, bytes, &err)), 0};
-    if (err) return err;
-    auto* st = A.take<SelState>(1);
-    auto* flag = A.take<int32_t>(n);
-    auto* pos = A.take<int32_t>(n);
-    auto* dense = A.take<double2>(n);
-    auto* tmp = A.take<int32_t>(tiles + 1);
-    auto* total = A.take<int32_t>(1);
-    if (int rc = run_predicate(f, evec, x, y, nx, ny, dx, dy, sdd_thresh, percentile, st, flag,
-                               dense, s))
-        return rc;
-    if (int rc = exclusive_scan(flag, n, pos, tmp, total, s)) return rc;
-    k_compact<<<blocks_for(n, 256), 256, 0, s>>>(flag, pos, dense, n,
-                                                 reinterpret_cast<double2*>(pts), nullptr, nullptr);
-    NBX_LAUNCHED("compact");
-    int32_t cnt = 0;
-    if (int rc = read_counts(total, 1, &cnt, s)) return rc;
-    *npts_host = cnt;
+// Count read-back without the pinned staging buffer (for callers that must
+// not allocate): a copy into pageable host memory, then a wait.
+int read_counts_pageable(const int32_t* dev, int count, int32_t* host, hipStream_t s) {
+    NBX_HIP(hipMemcpyAsync(host, dev, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
     return 0;
 }
 
-int nbx_ridges(const double* f, const double* evec, const double* x, int nx, const double* y,
+// ftle_ridges on a scratch arena of at least RidgeLayout::bytes(nx*ny)
+// bytes at `arena`; n = nx*ny > 0 and the arguments already checked.
+int ridges_run(const double* f, const double* evec, const double* x, int nx, const double* y,
                int ny, double dx, double dy, double sdd_thresh, double percentile,
                int min_ridge_pts, double* pts, int64_t* offsets, int64_t* nridges_host,
-               int64_t* npts_host, nbx_stream stream) {
-    if (int rc = check_grid(nx, ny)) return rc;
-    NBX_REQUIRE(nridges_host != nullptr && npts_host != nullptr, "NULL count pointer");
+               int64_t* npts_host, char* arena, bool pinned_counts, hipStream_t s) {
     const int64_t n = static_cast<int64_t>(nx) * ny;
-    *nridges_host = 0;
-    *npts_host = 0;
-    if (n == 0) return 0;
-    NBX_REQUIRE(f && evec && x && y && pts && offsets, "NULL array");
-    hipStream_t s = as_stream(stream);
-    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    int64_t N = kSortChunk;
-    while (N < n) N <<= 1;
-    struct Layout {
-        SelState* st;
-        int32_t *flag, *pos, *tmp, *count, *par, *idx, *seg_start, *seg_ex, *seg_first, *nseg,
-            *total, *nridges;  // nridges = total + 1: the pair scan's (points, ridges) total
-        I2 *kept, *out_off_idx, *tmp2;
-        double2* dense;
-        unsigned long long* keys;
-        void carve(Arena& A, int64_t n, int64_t tiles, int64_t N) {
-            st = A.take<SelState>(1);
-            flag = A.take<int32_t>(n);
-            pos = A.take<int32_t>(n);
-            dense = A.take<double2>(n);
-            tmp = A.take<int32_t>(tiles + 1);   // tile offsets: compaction, then segments
-            tmp2 = A.take<I2>(tiles + 1);       // tile offsets of the pair scan
-            count = A.take<int32_t>(1);
-            par = A.take<int32_t>(n);
-            idx = A.take<int32_t>(n);
-            keys = A.take<unsigned long long>(N);
-            seg_start = A.take<int32_t>(n);
-            seg_ex = A.take<int32_t>(n);
-            seg_first = A.take<int32_t>(n + 1);
-            kept = A.take<I2>(n);         // (kept length, kept flag) per segment
-            out_off_idx = A.take<I2>(n);  // their exclusive scans
-            nseg = A.take<int32_t>(1);
-            total = A.take<int32_t>(2);
-            nridges = total + 1;
-        }
-    } L;
-    Arena measure{nullptr, 0};
-    L.carve(measure, n, tiles, N);
-    int err = 0;
-    Arena A{static_cast<char*>(workspace(1, measure.used, &err)), 0};
-    if (err) return err;
-    L.carve(A, n, tiles, N);
+    auto read = [&](const int32_t* dev, int count, int32_t* host) {
+        return pinned_counts ? read_counts(dev, count, host, s)
+                             : read_counts_pageable(dev, count, host, s);
+    };
+    RidgeLayout L;
+    Arena A{arena, 0};
+    L.carve(A, n);
     SelState* st = L.st;
     int32_t *flag = L.flag, *pos = L.pos, *tmp = L.tmp, *count = L.count, *par = L.par,
             *idx = L.idx, *seg_start = L.seg_start, *seg_ex = L.seg_ex,
@@ -876,7 +839,7 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
         m = n - 1 - klo;
     } else {
         int32_t cnt = 0;
-        if (int rc = read_counts(count, 1, &cnt, s)) return rc;
+        if (int rc = read(count, 1, &cnt)) return rc;
         m = cnt;
     }
     if (m == 0) {
@@ -909,11 +872,83 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
                                         L.out_off_idx, L.tmp2, totals, dense, count, m,
                                         reinterpret_cast<double2*>(pts), offsets);
     NBX_LAUNCHED("group_scatter");
-    int32_t h[2] = {0, 0};  // total, nridges (adjacent in the arena)
-    if (int rc = read_counts(total, 2, h, s)) return rc;
+    int32_t h[3] = {0, 0, 0};  // points kept, ridges kept, ridge points (adjacent)
+    if (int rc = read(total, 3, h)) return rc;
+    // the grouping kernels clip to the capacity m; the percentile bound
+    // guarantees count <= m, and a violation is reported, not truncated
+    NBX_REQUIRE(h[2] <= m, "ridge-point count exceeds the grouping capacity");
     *nridges_host = h[1];
     *npts_host = h[0];
     return 0;
+}
+
+
+}  // namespace
+}  // namespace nbx
+
+using namespace nbx;
+
+extern "C" {
+
+int nbx_percentile(const double* f, int64_t n, double q, double* result, nbx_stream stream) {
+    NBX_REQUIRE(f != nullptr && result != nullptr, "NULL array");
+    hipStream_t s = as_stream(stream);
+    Workspace ws(s, 2, sizeof(SelState));
+    if (ws.error()) return ws.error();
+    return percentile_device(f, n, q, reinterpret_cast<SelState*>(ws.get()), result, s);
+}
+
+int nbx_ridge_pts(const double* f, const double* evec, const double* x, int nx, const double* y,
+                  int ny, double dx, double dy, double sdd_thresh, double percentile, double* pts,
+                  int64_t* npts_host, nbx_stream stream) {
+    if (int rc = check_grid(nx, ny)) return rc;
+    NBX_REQUIRE(npts_host != nullptr, "npts_host is NULL");
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    *npts_host = 0;
+    if (n == 0) return 0;
+    NBX_REQUIRE(f && evec && x && y && pts, "NULL array");
+    hipStream_t s = as_stream(stream);
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    const size_t bytes = align256(sizeof(SelState)) + align256(4 * n) * 2 + align256(16 * n) +
+                         align256(4 * (tiles + 1)) + align256(4);
+    Workspace ws(s, 1, bytes);
+    if (ws.error()) return ws.error();
+    Arena A{ws.get(), 0};
+    auto* st = A.take<SelState>(1);
+    auto* flag = A.take<int32_t>(n);
+    auto* pos = A.take<int32_t>(n);
+    auto* dense = A.take<double2>(n);
+    auto* tmp = A.take<int32_t>(tiles + 1);
+    auto* total = A.take<int32_t>(1);
+    if (int rc = run_predicate(f, evec, x, y, nx, ny, dx, dy, sdd_thresh, percentile, st, flag,
+                               dense, s))
+        return rc;
+    if (int rc = exclusive_scan(flag, n, pos, tmp, total, s)) return rc;
+    k_compact<<<blocks_for(n, 256), 256, 0, s>>>(flag, pos, dense, n,
+                                                 reinterpret_cast<double2*>(pts), nullptr, nullptr);
+    NBX_LAUNCHED("compact");
+    int32_t cnt = 0;
+    if (int rc = read_counts(total, 1, &cnt, s)) return rc;
+    *npts_host = cnt;
+    return 0;
+}
+
+int nbx_ridges(const double* f, const double* evec, const double* x, int nx, const double* y,
+               int ny, double dx, double dy, double sdd_thresh, double percentile,
+               int min_ridge_pts, double* pts, int64_t* offsets, int64_t* nridges_host,
+               int64_t* npts_host, nbx_stream stream) {
+    if (int rc = check_grid(nx, ny)) return rc;
+    NBX_REQUIRE(nridges_host != nullptr && npts_host != nullptr, "NULL count pointer");
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    *nridges_host = 0;
+    *npts_host = 0;
+    if (n == 0) return 0;
+    NBX_REQUIRE(f && evec && x && y && pts && offsets, "NULL array");
+    hipStream_t s = as_stream(stream);
+    Workspace ws(s, 1, RidgeLayout::bytes(n));
+    if (ws.error()) return ws.error();
+    return ridges_run(f, evec, x, nx, y, ny, dx, dy, sdd_thresh, percentile, min_ridge_pts, pts,
+                      offsets, nridges_host, npts_host, ws.get(), true, s);
 }
 
 int nbx_label8(const uint8_t* b, int nx, int ny, int32_t* labels, int32_t* nlabels_host,
@@ -927,9 +962,9 @@ int nbx_label8(const uint8_t* b, int nx, int ny, int32_t* labels, int32_t* nlabe
     hipStream_t s = as_stream(stream);
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
     const size_t bytes = align256(4 * n) * 4 + align256(4 * (tiles + 1)) + align256(4);
-    int err = 0;
-    Arena A{static_cast<char*>(workspace(1, bytes, &err)), 0};
-    if (err) return err;
+    Workspace ws(s, 1, bytes);
+    if (ws.error()) return ws.error();
+    Arena A{ws.get(), 0};
     auto* flag = A.take<int32_t>(n);
     auto* par = A.take<int32_t>(n);
     auto* isroot = A.take<int32_t>(n);
@@ -974,9 +1009,9 @@ int nbx_ordered_ridges(const double* f, const double* evec, const double* x, int
     hipStream_t s = as_stream(stream);
     const size_t bytes = align256(sizeof(SelState)) + align256(4 * n) + align256(16 * n) +
                          align256(8 * n);
-    int err = 0;
-    Arena A{static_cast<char*>(workspace(1, bytes, &err)), 0};
-    if (err) return err;
+    Workspace ws(s, 1, bytes);
+    if (ws.error()) return ws.error();
+    Arena A{ws.get(), 0};
     auto* st = A.take<SelState>(1);
     auto* flag = A.take<int32_t>(n);
     auto* dense = A.take<double2>(n);

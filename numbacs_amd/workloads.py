@@ -75,18 +75,28 @@ def fourier_modes(seed=FIELD_SEED, nmodes=64):
 
 
 def synthetic_velocity(nt=96, seed=FIELD_SEED, rms_kmh=30.0):
-    """(t, lon, lat, U, V): U, V of shape (nt, 271, 101) in km/h, 'ij' order."""
+    """(t, lon, lat, U, V): U, V of shape (nt, 271, 101) in km/h, 'ij' order.
+
+    psi = sum_m amp_m cos(B_m(x, y) - omega_m t) with B_m = 2 pi (kx x + ky y)
+    + phase_m; u = -dpsi/dlat and v = dpsi/dlon (per degree) are sums of
+    amp_m k_m sin(B_m - omega_m t) = amp_m k_m (sin B_m cos omega_m t -
+    cos B_m sin omega_m t), formed as two (nt x modes) @ (modes x points)
+    products."""
     t, lon, lat = field_axes(nt)
     Lx, Ly = lon[-1] - lon[0], lat[-1] - lat[0]
-    T, X, Y = np.meshgrid(t, (lon - lon[0]) / Lx, (lat - lat[0]) / Ly, indexing="ij")
-    U = np.zeros(T.shape)
-    V = np.zeros(T.shape)
-    for kx, ky, amp, phase, omega in fourier_modes(seed):
-        th = 2 * math.pi * (kx * X + ky * Y) - omega * T + phase
-        s = np.sin(th)
-        # psi = amp * cos(th): u = -dpsi/dlat, v = dpsi/dlon (per degree)
-        U += amp * s * (2 * math.pi * ky / Ly)
-        V += -amp * s * (2 * math.pi * kx / Lx)
+    X, Y = np.meshgrid((lon - lon[0]) / Lx, (lat - lat[0]) / Ly, indexing="ij")
+    modes = fourier_modes(seed)
+    B = np.stack([2 * math.pi * (kx * X + ky * Y) + phase for kx, ky, _, phase, _ in modes])
+    B = B.reshape(len(modes), -1)
+    SB, CB = np.sin(B), np.cos(B)
+    om = np.array([m[4] for m in modes])
+    Tc, Ts = np.cos(np.outer(t, om)), np.sin(np.outer(t, om))
+    amp = np.array([m[2] for m in modes])
+    cu = amp * np.array([2 * math.pi * m[1] / Ly for m in modes])
+    cv = -amp * np.array([2 * math.pi * m[0] / Lx for m in modes])
+    shape = (nt,) + X.shape
+    U = ((Tc * cu) @ SB - (Ts * cu) @ CB).reshape(shape)
+    V = ((Tc * cv) @ SB - (Ts * cv) @ CB).reshape(shape)
     scale = rms_kmh / math.sqrt(np.mean(U ** 2 + V ** 2))
     return t, lon, lat, U * scale, V * scale
 

@@ -55,6 +55,23 @@ def ridge_pts_data():
 
 
 @pytest.fixture
+def ridge_data():
+    """The reference's ridges.pkl (tests/conftest.py:134-138 of the
+    reference) as data: three ridges of 6, 9 and 3 points.  Converted by
+    tests/golden/make_pickle_fixtures.py without unpickling."""
+    d = np.load(golden("ridges.npz"))
+    return [d[f"r{k}"] for k in range(len(d.files))]
+
+
+@pytest.fixture
+def ordered_ridge_data():
+    """The reference's ordered_ridges.pkl (tests/conftest.py:140-144 of the
+    reference): two ordered curves of 6 and 7 points at dist_tol 0.1."""
+    d = np.load(golden("ordered_ridges.npz"))
+    return [d[f"r{k}"] for k in range(len(d.files))]
+
+
+@pytest.fixture
 def fm_aux_data():
     return np.load(golden("fm_aux.npy"))
 
@@ -130,3 +147,33 @@ def flowmap_agreement(fm, ref, sens, rtol=FLOWMAP_RTOL, amplification=1e4):
     err = np.abs(fm - ref).max(-1)
     ok = bool(np.all(err <= np.maximum(rtol * scale, amplification * sens)))
     return ok, float(np.mean(err <= rtol * scale)), err
+
+
+def _grad(fm, dx, dy):
+    """Central-difference flow-map gradient on the interior (utils.py:9-46
+    of the reference): (nx-2, ny-2, 2, 2), G[..., a, b] = dF_a / dX_b."""
+    G = np.empty((fm.shape[0] - 2, fm.shape[1] - 2, 2, 2))
+    G[..., :, 0] = (fm[2:, 1:-1] - fm[:-2, 1:-1]) / (2 * dx)
+    G[..., :, 1] = (fm[1:-1, 2:] - fm[1:-1, :-2]) / (2 * dy)
+    return G
+
+
+def ftle_bound(fm_a, fm_b, T, dx, dy, slack=1e-12):
+    """Per-point bound on |FTLE(fm_a) - FTLE(fm_b)| that the difference of the
+    two flow maps implies (interior points; 0 on the border).
+
+    FTLE = max(log sigma, 0) / |T| with sigma the largest singular value of
+    the gradient G.  By Weyl's inequality |sigma_a - sigma_b| <= ||G_a -
+    G_b||_2 <= ||G_a - G_b||_F, and |log a - log b| <= |a - b| / min(a, b);
+    max(., 0) is 1-Lipschitz.  slack covers the stencil's own rounding
+    (relative to the FTLE)."""
+    Ga, Gb = _grad(fm_a, dx, dy), _grad(fm_b, dx, dy)
+    dG = np.sqrt(((Ga - Gb) ** 2).sum(axis=(-1, -2)))
+    sa = np.linalg.svd(Ga, compute_uv=False)[..., 0]
+    sb = np.linalg.svd(Gb, compute_uv=False)[..., 0]
+    out = np.zeros(fm_a.shape[:2])
+    lo = np.minimum(sa, sb)
+    b = np.where(lo > 0, dG / np.maximum(lo, 1e-300), np.inf) / abs(T)
+    b += slack * np.maximum(np.log(np.maximum(sa, 1.0)), np.log(np.maximum(sb, 1.0))) / abs(T)
+    out[1:-1, 1:-1] = b
+    return out

@@ -49,6 +49,22 @@ def test_ordered_ridges_gpu_vs_oracle(case, coords_dg, ftle_data, C_eig_data):
     _check(got, want)
 
 
+def test_ordered_ridges_reference_fixture(coords_dg, ftle_data, C_eig_data, ordered_ridge_data):
+    """tests/test_extraction.py:22-28 of the reference on the GPU path: two
+    curves of 6 and 7 points equal to ordered_ridges.pkl in order and
+    direction (within 2e-9: the pickle predates the float32 storage of its
+    inputs)."""
+    from numbacs_amd.extraction import ftle_ordered_ridges
+
+    x, y = coords_dg
+    f = ftle_data.astype(np.float64)
+    ev = C_eig_data[1][:, :, :, 1].astype(np.float64)
+    got = ftle_ordered_ridges(f, ev, x, y, 0.1)
+    assert [len(a) for a in got] == [len(a) for a in ordered_ridge_data] == [6, 7]
+    for a, b in zip(got, ordered_ridge_data):
+        assert np.abs(a - b).max() < 2e-9
+
+
 def test_ordered_ridges_gridded_workload_structure():
     """On the gridded workload's FTLE field: every curve is an ordered chain
     of neighbouring ridge points or joined pieces within dist_tol, the curves

@@ -339,9 +339,11 @@ def test_ridge_pts_fixture(nb, coords_dg, ftle_data, C_eig_data, ridge_pts_data)
     assert np.array_equal(pts, ref)  # the predicate runs uncontracted in source order
 
 
-def test_ridges_fixture_grouping(nb, coords_dg, ftle_data, C_eig_data):
+def test_ridges_fixture_grouping(nb, coords_dg, ftle_data, C_eig_data, ridge_data):
     """ftle_ridges on the fixture world: grouping equals scipy.ndimage.label's
-    (the reference's dependency) applied to the oracle's ridge flags."""
+    (the reference's dependency) applied to the oracle's ridge flags, and the
+    ridges equal the reference's ridges.pkl (tests/test_extraction.py:14-19;
+    within 2e-9: the pickle predates the float32 storage of its inputs)."""
     from numbacs_amd.extraction import ftle_ridges
 
     x, y = coords_dg
@@ -356,6 +358,8 @@ def test_ridges_fixture_grouping(nb, coords_dg, ftle_data, C_eig_data):
             expect.append(pts[inds])
     assert [len(r) for r in ridges] == [len(r) for r in expect] == [6, 9, 3]
     assert all(np.array_equal(a, b) for a, b in zip(ridges, expect))
+    assert [len(r) for r in ridge_data] == [6, 9, 3]
+    assert all(np.abs(a - b).max() < 2e-9 for a, b in zip(ridges, ridge_data))
 
 
 @pytest.mark.parametrize("q", [0, 50, 90, 99.5, 100])
@@ -471,6 +475,68 @@ def test_ridges_grouping_beyond_one_workgroup(nb, percentile):
         assert np.array_equal(a, b)
 
 
+def test_ridges_grouping_many_segments(nb):
+    """More than 1024 components (one scan tile of segments): 95 ridge lines
+    cut into random dashes by zeroed rows, about 8,000 components of which
+    min_ridge_pts drops the short ones, so kept and dropped segments span
+    many segment-scan tiles; against the oracle."""
+    from numbacs_amd.extraction import ftle_ridges
+
+    nx, ny = 1200, 300
+    x, y = np.linspace(0, 3, nx), np.linspace(-1, 1, ny)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    f = 2 + np.cos(200 * X + 0.3 * np.sin(3 * Y)) + 0.1 * np.cos(7 * Y)
+    gap = np.random.default_rng(5).random(ny) < 0.3
+    f[:, gap] = 0.0
+    ev = np.zeros((nx, ny, 2))
+    ev[..., 0] = 1.0
+    ridges = ftle_ridges(f, ev, x, y, min_ridge_pts=3)
+    ridges_ref = oracle.ridges(f, ev, x, y, min_ridge_pts=3)
+    _, rb = oracle.ridge_predicate(f, ev, x, y)
+    _, ncomp = oracle.label8(rb)
+    assert ncomp > 4 * 1024 and 1024 < len(ridges_ref) < ncomp
+    assert len(ridges) == len(ridges_ref)
+    for a, b in zip(ridges, ridges_ref):
+        assert np.array_equal(a, b)
+
+
+def test_ridges_two_streams_concurrently(nb):
+    """ftle_ridges / percentile calls on two streams from two host threads
+    at once (each stream has its own workspace) give the sequential results."""
+    import threading
+
+    import torch
+
+    from numbacs_amd.extraction import ftle_ridges
+
+    cases = [(_field_and_evec(301, 211, 1), 90), (_field_and_evec(257, 389, 2), 0)]
+    seq = [ftle_ridges(f, ev, x, y, percentile=q) for (f, ev, x, y), q in cases]
+    assert all(len(r) > 3 for r in seq)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    got = [[], []]
+    errors = []
+
+    def work(k):
+        try:
+            (f, ev, x, y), q = cases[k]
+            with torch.cuda.stream(streams[k]):
+                for _ in range(6):
+                    got[k].append(ftle_ridges(f, ev, x, y, percentile=q))
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    for k in range(2):
+        assert len(got[k]) == 6
+        for r in got[k]:
+            assert len(r) == len(seq[k]) and all(np.array_equal(a, b) for a, b in zip(r, seq[k]))
+
+
 def test_ridges_empty_and_degenerate(nb):
     from numbacs_amd.extraction import ftle_ridge_pts, ftle_ridges
 
@@ -580,3 +646,41 @@ def test_particles_stop_at_grid_edge(nb, precision):
     want = np.maximum(x0 - 0.5 * 40.0, 0.0)
     assert np.abs(fm[..., 0] - want[:, None]).max() < 1e-3
     assert np.abs(fm[..., 1] - y0[None, :]).max() < 1e-12
+
+
+# --------------------------------------------- spherical conversion --
+
+
+@pytest.mark.parametrize("spherical", [1, 2])
+@pytest.mark.parametrize("a", [0.0, 0.5])
+@pytest.mark.parametrize("T", [24.0, -48.0])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_spherical_conversion_closed_form(nb, spherical, a, T, precision):
+    """flows.py:154-239 on the GPU: the trajectories of tests/spherical_cases.py
+    (rhumb lines of a uniform field; a zonal profile U0 (1 + a sin lon)
+    crossing the +-180 / 0-360 wrap) against their closed form.  FP64 to the
+    oracle's own bar (rtol 1e-10); the FP32-displacement form (float spline
+    and conversion; float stages cannot resolve rtol 1e-10, so it runs at the
+    default rtol 1e-6) to 1e-5 relative in the lon integral (1e-4 deg for the
+    uniform field) and 1e-5 deg in lat."""
+    import spherical_cases as SC
+    from numbacs_amd import _lib
+    from numbacs_amd.flows import get_flow_2D, get_interp_arrays_2D
+    from numbacs_amd.integration import flowmap_grid_2d_device
+
+    V0 = 20.0
+    t, lon, lat, U, V = SC.field(spherical, a, 40.0, V0)
+    grid, Cu, Cv = get_interp_arrays_2D(t, lon, lat, U, V)
+    f = get_flow_2D(grid, Cu, Cv, spherical=spherical)
+    lon0, lat0 = SC.particles(spherical)
+    st = _lib.empty((len(lon0), len(lat0)), "i32")
+    tol = dict(rtol=1e-10, atol=1e-12) if precision == 0 else dict(rtol=1e-6, atol=1e-8)
+    out = flowmap_grid_2d_device(f, SC.T0, T, _lib.to_dev(lon0), _lib.to_dev(lat0),
+                                 np.array([math.copysign(1.0, T)]), precision=precision,
+                                 status=st, **tol)
+    fm = _lib.to_host(out)
+    assert (_lib.to_host(st) == 1).all()
+    if precision == 0:
+        SC.check(fm, spherical, a, T, 40.0, V0, 1e-8, 1e-9)
+    else:
+        SC.check(fm, spherical, a, T, 40.0, V0, 1e-5 if a else 1e-4, 1e-5)

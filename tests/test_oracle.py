@@ -4,6 +4,7 @@ the checks that pin it before it is trusted as the checker of the HIP path.
 Planted mistakes (in the oracle's inputs, on the CPU) show the checks bite.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -227,10 +228,39 @@ def test_ridge_pts_fixture(coords_dg, ftle_data, C_eig_data, ridge_pts_data):
     assert pts.shape == ridge_pts_data.shape and np.allclose(pts, ridge_pts_data)
 
 
-def test_ridges_fixture_counts(coords_dg, ftle_data, C_eig_data):
+def test_ridges_fixture(coords_dg, ftle_data, C_eig_data, ridge_data):
+    """ftle_ridges on the fixture world equals ridges.pkl: same ridges, same
+    order, same points (tests/test_extraction.py:14-19 of the reference).
+    The pickle holds the points the reference computed before its inputs were
+    stored as float32, so they agree to 5.5e-10 here, inside the reference's
+    own np.allclose bar (atol 1e-8)."""
     x, y = coords_dg
     r = oracle.ridges(ftle_data, C_eig_data[1][:, :, :, 1], x, y)
-    assert [len(a) for a in r] == [6, 9, 3]
+    assert [len(a) for a in r] == [len(a) for a in ridge_data] == [6, 9, 3]
+    for got, want in zip(r, ridge_data):
+        assert np.abs(got - want).max() < 2e-9
+
+
+def test_pickle_fixture_reader_refuses_foreign_globals(tmp_path):
+    """The fixture converter accepts numpy's array reconstruct globals only:
+    a stream naming any other callable is rejected before anything runs."""
+    import importlib.util
+    import pickle
+
+    spec = importlib.util.spec_from_file_location(
+        "make_pickle_fixtures", os.path.join(os.path.dirname(__file__), "golden",
+                                             "make_pickle_fixtures.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    good = tmp_path / "good.pkl"
+    arrs = [np.arange(6.0).reshape(3, 2), np.array([[1.5, -2.0]])]
+    good.write_bytes(pickle.dumps(arrs, protocol=4))
+    back = mod.read_array_list(str(good))
+    assert all(np.array_equal(a, b) for a, b in zip(arrs, back))
+    bad = tmp_path / "bad.pkl"
+    bad.write_bytes(pickle.dumps([os.getcwd], protocol=4))
+    with pytest.raises(ValueError):
+        mod.read_array_list(str(bad))
 
 
 @pytest.mark.parametrize("q", [0, 10, 50, 90, 99.9, 100])
@@ -416,3 +446,70 @@ def test_trilinear_exact_and_extrapolation():
     assert np.array_equal(oracle.linear3_eval(grid, F, qo, "constant"), np.zeros(len(q)))
     assert np.abs(oracle.linear3_eval(grid, F, qo, "nearest") - f(q[:, 0], q[:, 1], 2.0)).max() < 1e-12
     assert np.abs(oracle.linear3_eval(grid, F, qo, "linear") - f(*qo.T)).max() < 1e-11
+
+
+# ------------------------------------------------ spherical conversion --
+
+SPH_TOL = dict(lon_tol_uniform=1e-8, lon_tol_sin=1e-8, lat_tol=1e-9)
+
+
+def _sph_oracle(spherical, a, T, U0, V0, r=6371.0, field_spherical=None):
+    import spherical_cases as SC
+
+    t, lon, lat, U, V = SC.field(field_spherical or spherical, a, U0, V0)
+    grid = ((t[0], t[-1], len(t)), (lon[0], lon[-1], len(lon)), (lat[0], lat[-1], len(lat)))
+    S = oracle.Spline(grid, oracle.prefilter3(U), oracle.prefilter3(V), spherical=spherical, r=r)
+    lon0, lat0 = SC.particles(spherical)
+    fm, st, _ = oracle.flowmap_grid_2d("spline", np.array([math.copysign(1.0, T)]), SC.T0, T,
+                                       lon0, lat0, rtol=1e-10, atol=1e-12, spline=S)
+    assert (st == 1).all()
+    return fm
+
+
+@pytest.mark.parametrize("spherical", [1, 2])
+@pytest.mark.parametrize("a", [0.0, 0.5])
+@pytest.mark.parametrize("V0", [0.0, 20.0])
+@pytest.mark.parametrize("T", [24.0, -48.0])
+def test_spherical_conversion_closed_form(spherical, a, V0, T):
+    """flows.py:154-239: km/h -> deg/h with 180/(pi r cos lat) and 180/(pi r),
+    the +-180 (spherical=1) and 0/360 (spherical=2) wraps of the lookup, for
+    both time directions: every trajectory matches the closed form of
+    tests/spherical_cases.py (rhumb lines; a separable zonal profile)."""
+    import spherical_cases as SC
+
+    fm = _sph_oracle(spherical, a, T, 40.0, V0)
+    SC.check(fm, spherical, a, T, 40.0, V0, SPH_TOL["lon_tol_sin" if a else "lon_tol_uniform"],
+             SPH_TOL["lat_tol"])
+
+
+@pytest.mark.parametrize("plant", ["radius", "wrap"])
+def test_spherical_planted_mistakes_are_caught(plant):
+    """A radius off by 1e-4, or the other wrap convention, fails the check."""
+    import spherical_cases as SC
+
+    if plant == "radius":
+        fm = _sph_oracle(1, 0.5, -48.0, 40.0, 20.0, r=6371.0 * (1 + 1e-4))
+    else:
+        fm = _sph_oracle(1, 0.5, -48.0, 40.0, 20.0, field_spherical=2)
+    with pytest.raises(AssertionError):
+        SC.check(fm, 1, 0.5, -48.0, 40.0, 20.0, SPH_TOL["lon_tol_sin"], SPH_TOL["lat_tol"])
+
+
+def test_ftle_bound_holds_and_is_tight():
+    """conftest.ftle_bound: the FTLE change caused by a flow-map perturbation
+    stays within the bound, and the bound is within 10x of the change where
+    the change is large (it is a first-order bound, not a blanket)."""
+    from conftest import ftle_bound
+
+    x, y = np.linspace(0, 2, 101), np.linspace(0, 1, 51)
+    fm, _, _ = oracle.flowmap_grid_2d(DG, dg_params(-1.0), 0.0, -10.0, x, y)
+    dx, dy = x[1] - x[0], y[1] - y[0]
+    rng = np.random.default_rng(3)
+    fm2 = fm + 1e-7 * rng.standard_normal(fm.shape)
+    f1 = oracle.ftle_grid_2d(fm, -10.0, dx, dy)
+    f2 = oracle.ftle_grid_2d(fm2, -10.0, dx, dy)
+    b = ftle_bound(fm, fm2, -10.0, dx, dy)
+    d = np.abs(f1 - f2)
+    assert (d <= b).all()
+    big = d > 1e-3 * d.max()
+    assert np.median(b[big] / d[big]) < 10

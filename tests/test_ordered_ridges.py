@@ -2,8 +2,10 @@
 ridges whose answer is known in closed form, and the library's native
 linking (nbx_ordered_ridges_link, host code, no GPU) against it bit for bit.
 
-No reference fixture pins this function: the reference's ordered_ridges.pkl
-was refused by the safe loader and is not used.
+The reference's ordered_ridges.pkl (converted to data by
+tests/golden/make_pickle_fixtures.py) pins the exact sequence on the fixture
+world: two curves of 6 and 7 points at dist_tol 0.1, in the reference's
+order and direction.
 """
 import ctypes
 import math
@@ -121,6 +123,24 @@ def test_circle_is_one_closed_curve():
 
 def _same(a, b):
     return len(a) == len(b) and all(np.array_equal(u, v) for u, v in zip(a, b))
+
+
+def test_oracle_matches_reference_fixture(coords_dg, ftle_data, C_eig_data,
+                                          ordered_ridge_data):
+    """tests/test_extraction.py:22-28 of the reference: ftle_ordered_ridges
+    on the fixture world at dist_tol 0.1 (defaults otherwise) gives the two
+    curves of ordered_ridges.pkl, point for point and in the same direction
+    (within 2e-9: the pickle predates the float32 storage of its inputs);
+    the native linking gives the same."""
+    x, y = coords_dg
+    f = ftle_data.astype(np.float64)
+    ev = C_eig_data[1][:, :, :, 1].astype(np.float64)
+    want = ordered_ridge_data
+    assert [len(r) for r in want] == [6, 7]
+    for got in (O.ordered_ridges(f, ev, x, y, 0.1), product(f, ev, x, y, 0.1)):
+        assert [len(r) for r in got] == [6, 7]
+        for g, w in zip(got, want):
+            assert np.abs(g - w).max() < 2e-9
 
 
 @pytest.mark.parametrize("case", ["straight", "gap", "circle", "fixture", "random"])
