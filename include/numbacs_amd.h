@@ -263,6 +263,21 @@ int nbx_ridges(const double* f, const double* eigvec_max, const double* x, int n
                double percentile, int min_ridge_pts, double* pts, int64_t* offsets,
                int64_t* nridges_host, int64_t* npts_host, nbx_stream stream);
 
+/* nbx_ridges without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and errors as nbx_ridges, but the pipeline's scratch is a fixed 64 MiB
+ * device array of the library (one per GPU, held until the process ends)
+ * instead of an allocated workspace, and the counts are read back without
+ * the pinned staging buffer; nothing is allocated.  Grids whose scratch
+ * exceeds 64 MiB (about 980,000 points) are rejected with an error.  All
+ * calls on one GPU share that array: calls that overlap in time on one GPU
+ * (two streams or two host threads) race. */
+int nbx_ridges_noalloc(const double* f, const double* eigvec_max, const double* x, int nx,
+                       const double* y, int ny, double dx, double dy, double sdd_thresh,
+                       double percentile, int min_ridge_pts, double* pts, int64_t* offsets,
+                       int64_t* nridges_host, int64_t* npts_host, nbx_stream stream);
+
 /* 8-connected labelling of a byte grid (scipy.ndimage.label with a full
  * 3x3 structure, ridges.py:221-222): labels (nx*ny int32, 0 = background,
  * 1.. in raster order of first pixel); *nlabels_host the count.

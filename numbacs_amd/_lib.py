@@ -77,6 +77,8 @@ SIGNATURES = {
                              _c_p, _c_p]),
     "nbx_ridges": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_d, _c_d, _c_i, _c_p,
                           _c_p, _c_p, _c_p, _c_p]),
+    "nbx_ridges_noalloc": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_d, _c_d, _c_i,
+                                  _c_p, _c_p, _c_p, _c_p, _c_p]),
     "nbx_ordered_ridges": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_i, _c_d,
                                   _c_d, _c_d, _c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p]),
     "nbx_ordered_ridges_link": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_d, _c_d, _c_i,

@@ -29,9 +29,19 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "common.h"
 #include "ordered_ridges.h"
+
+// Scratch of nbx_ridges_noalloc: a fixed 64 MiB device array in place of the
+// allocated workspace, for callers that cannot allocate.  The host never
+// uses the array's name as an address: each device's copy lives at its own
+// address, which hipGetSymbolAddress returns (once per device, with that
+// device current, in scratch_address below) and which is then passed to the
+// kernels.
+static __device__ __align__(256) char scratch[static_cast<size_t>(64) << 20];
 
 namespace nbx {
 
@@ -741,6 +751,26 @@ int check_grid(int nx, int ny) {
     return 0;
 }
 
+constexpr size_t kNoAllocScratchBytes = sizeof(scratch);  // 64 MiB
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+char* g_scratch_addr[kMaxDevices] = {};  // per-device address of `scratch`
+
+int scratch_address(char** out) {
+    int dev = 0;
+    NBX_HIP(hipGetDevice(&dev));
+    NBX_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_addr[dev] == nullptr) {
+        void* p = nullptr;
+        NBX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        NBX_REQUIRE(p != nullptr, "hipGetSymbolAddress returned NULL for the scratch array");
+        g_scratch_addr[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_addr[dev];
+    return 0;
+}
+
 // Scratch layout of the grouped-ridge pipeline for n grid points.
 struct RidgeLayout {
     SelState* st;
@@ -949,6 +979,33 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     if (ws.error()) return ws.error();
     return ridges_run(f, evec, x, nx, y, ny, dx, dy, sdd_thresh, percentile, min_ridge_pts, pts,
                       offsets, nridges_host, npts_host, ws.get(), true, s);
+}
+
+int nbx_ridges_noalloc(const double* f, const double* evec, const double* x, int nx,
+                       const double* y, int ny, double dx, double dy, double sdd_thresh,
+                       double percentile, int min_ridge_pts, double* pts, int64_t* offsets,
+                       int64_t* nridges_host, int64_t* npts_host, nbx_stream stream) {
+    if (int rc = check_grid(nx, ny)) return rc;
+    NBX_REQUIRE(nridges_host != nullptr && npts_host != nullptr, "NULL count pointer");
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    *nridges_host = 0;
+    *npts_host = 0;
+    if (n == 0) return 0;
+    NBX_REQUIRE(f && evec && x && y && pts && offsets, "NULL array");
+    const size_t need = RidgeLayout::bytes(n);
+    if (need > kNoAllocScratchBytes) {
+        set_error("nbx_ridges_noalloc: a grid of " + std::to_string(n) + " points needs " +
+                  std::to_string(need) + " bytes of scratch, above its fixed " +
+                  std::to_string(kNoAllocScratchBytes) + " (use nbx_ridges)");
+        return -2;
+    }
+    char* base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    // every buffer the pipeline reads is written earlier in the same call
+    // (counters and histograms are cleared by its own kernels), so the
+    // previous call's contents need no clearing
+    return ridges_run(f, evec, x, nx, y, ny, dx, dy, sdd_thresh, percentile, min_ridge_pts, pts,
+                      offsets, nridges_host, npts_host, base, false, as_stream(stream));
 }
 
 int nbx_label8(const uint8_t* b, int nx, int ny, int32_t* labels, int32_t* nlabels_host,

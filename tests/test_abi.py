@@ -73,3 +73,19 @@ def test_batch_and_launch_config_validation_without_gpu():
     assert lib.nbx_mlp_gemm(0, None, None, 65, 64, 32, None, None, None, None, None) != 0
     assert b"multiples" in lib.nbx_last_error()
     lib.nbx_flow_destroy(flow)
+
+
+def test_noalloc_ridges_rejects_grids_beyond_its_scratch():
+    """nbx_ridges_noalloc refuses a grid whose scratch exceeds its fixed
+    array on the host, before any device call (dummy device addresses are
+    never touched)."""
+    from numbacs_amd import _lib
+
+    lib = _lib.load()
+    nr, npt = ctypes.c_int64(), ctypes.c_int64()
+    dummy = ctypes.c_void_p(256)
+    rc = lib.nbx_ridges_noalloc(dummy, dummy, dummy, 1000, dummy, 1000, 1.0, 1.0, 0.0, 90.0, 3,
+                                dummy, dummy, ctypes.byref(nr), ctypes.byref(npt), None)
+    assert rc != 0
+    assert b"scratch" in lib.nbx_last_error() and b"nbx_ridges" in lib.nbx_last_error()
+    assert nr.value == 0 and npt.value == 0
