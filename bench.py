@@ -54,12 +54,14 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-timeseries", action="store_true",
+                    help="default workload: leave out the time-series legs")
     ap.add_argument("--launch", default=None, metavar="LANES,BLOCK,PERSIST",
                     help="gridded workload: fixed flow-map launch shape instead of the "
                          "autotuner (e.g. 4,128,0)")
     ap.add_argument("--workload", default="gridded",
                     choices=["gridded", "double_gyre", "timeseries", "precision", "surrogate",
-                             "composition", "next", "copycal"])
+                             "composition", "next", "copycal", "stencil"])
     ap.add_argument("--surrogate-samples", type=int, default=4_000_000)
     ap.add_argument("--surrogate-t0s", type=int, default=2)
     return ap.parse_args()
@@ -167,12 +169,7 @@ def bench_gridded(args):
     achieved_tflops = flops_launch / (fm_ms * 1e-3) / 1e12
     _, _, nridges, nridge_pts = res
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "flowmap_spline_gridded_676x251_hbm.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
-
+    hbm = hbm_traffic(fm_ms)
     rec = {
         "metric": "FTLE grid points/s (gridded-velocity FTLE + ridges, plain FP64 DOP853)",
         "value": P / (wall / args.steps),
@@ -197,20 +194,24 @@ def bench_gridded(args):
                        "persistent_blocks_per_cu": tuned[2]},
         },
         "roofline": {
-            "bound": "hbm",
-            "kernel": "DOP853 flow map over the tricubic spline (k_flowmap / k_flowmap_queue)",
-            "achieved": achieved_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": traffic,
+            "bound": "fp64",
+            "kernel": "k_flowmap<Spline3<double,4>> (DOP853 over the tricubic spline, 4 lanes "
+                      "per particle)",
+            "achieved": achieved_tflops,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "traffic": hbm["bytes_per_launch"] if hbm else None,
             "kernel_ms": fm_ms,
-            "algorithmic_bytes_per_launch": bytes_launch,
-            "algorithmic_unit": "12,288 B per attempted step of the particles that end inside "
-                                "the velocity box (lower bound: exiting particles gather until "
-                                "they leave)",
-            "fp64_tflops": achieved_tflops,
-            "fp64_frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "algorithmic_flops_per_launch": flops_launch,
+            "algorithmic_unit": f"{FLOPS_PER_PARTICLE_STEP} FP64 flop per attempted step "
+                                "(350 solver + 12 x 400 per spline evaluation) of the particles "
+                                "that end inside the velocity box (a lower bound)",
+            "hbm": hbm,
+            "gather": {"bytes_per_launch": bytes_launch,
+                       "achieved_GBs": achieved_gbs,
+                       "unit": "12,288 B of spline coefficients per attempted step, served by "
+                               "L1/L2 (not an HBM figure)"},
         },
         "particle_steps_per_s": steps_total / (fm_ms * 1e-3),
         "mean_steps_per_particle": steps_total / P,
@@ -221,15 +222,136 @@ def bench_gridded(args):
         "autotune_ms": tune_ms,
     }
     if not args.no_cpu:
-        rec["cpu_baseline"], rec["error_vs_fp64_oracle"] = cpu_baseline(W, xp, yp, fm_h, ftle_h,
-                                                                         t0, T, args.cpu_reps)
+        rec["cpu_baseline"], rec["error_vs_fp64_oracle"] = cpu_baseline(
+            W, xp, yp, fm_h, ftle_h, _lib.to_host(evec), t0, T, args.cpu_reps)
+    if not args.no_timeseries:
+        print("bench: time-series legs", file=sys.stderr, flush=True)
+        rec["timeseries"] = timeseries_legs(tune=not args.no_tune)
     return rec
 
 
-def cpu_baseline(W, xp, yp, fm_gpu, ftle_gpu, t0, T, reps):
+def hbm_traffic(kernel_ms):
+    """HBM bytes per launch of the spline flow map from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE record (profiles/), scaled by the
+    device-copy calibration stored with it; None if the record is absent."""
+    tpath = os.path.join(ROOT, "profiles", "flowmap_spline_gridded_676x251_hbm.json")
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as fh:
+        rec = json.load(fh)
+    b = rec.get("hbm_bytes_per_launch")
+    if b is None:
+        return None
+    gbs = b / (kernel_ms * 1e-3) / 1e9
+    return {"bytes_per_launch": b, "achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS,
+            "frac": gbs / HBM_PEAK_GBS,
+            "source": "profiles/flowmap_spline_gridded_676x251_hbm.json (rocprofv3 --pmc "
+                      "FETCH_SIZE / WRITE_SIZE of this kernel, not measured in this run)",
+            "measured_at_commit": rec.get("measured_at_commit")}
+
+
+def timeseries_legs(reps=3, tune=True, oracle_rows=16):
+    """The FTLE time series (48 uneven requests over the 480-frame field) in
+    three forms: the plain FP64 path (one flowmap_grid_2D + ftle_grid_2D per
+    request, as the reference's t0 loop), the batched FP64 form and the
+    batched FP32-displacement form (one persistent work-queue launch plus one
+    batched FTLE launch, HIP graph replay).  FTLE of every request is compared
+    with the FP64 oracle on every oracle_rows-th particle row (the oracle
+    integrates each sampled row and its two neighbours)."""
+    import torch
+
+    import oracle
+    from numbacs_amd import _lib, autotune
+    from numbacs_amd.batch import FTLEBatch
+    from numbacs_amd.diagnostics import ftle_grid_2d_device
+    from numbacs_amd.integration import flowmap_grid_2d_device
+    from numbacs_amd.workloads import time_series_requests
+
+    W = gridded_field(480)
+    reqs = time_series_requests()
+    P = sum(len(q["x"]) * len(q["y"]) for q in reqs)
+    out = {"workload": "FTLE time series: 48 requests (t0 = 72..119 h hourly, T cycling "
+                       "-24/-48/-72 h, seeded grids 201x101..1351x501) over the 64-mode field "
+                       "extended to 271x101x480, spherical=1, DOP853 rtol=1e-6 atol=1e-8",
+           "particles": P}
+    # plain FP64: one single-grid launch + FTLE per request
+    plain = []
+    for q in reqs:
+        nx, ny = len(q["x"]), len(q["y"])
+        plain.append(dict(q, xd=_lib.to_dev(q["x"]), yd=_lib.to_dev(q["y"]),
+                          fm=_lib.empty((nx, ny, 2)), ftle=_lib.empty((nx, ny)),
+                          dx=float(q["x"][1] - q["x"][0]), dy=float(q["y"][1] - q["y"][0])))
+    single_cfg = autotune.get(_lib.NBX_FLOW_SPLINE3, _lib.NBX_FP64)
+
+    def run_plain():
+        for r in plain:
+            flowmap_grid_2d_device(W["flow"], r["t0"], r["T"], r["xd"], r["yd"], [-1.0],
+                                   out=r["fm"])
+            ftle_grid_2d_device(r["fm"], r["T"], r["dx"], r["dy"], out=r["ftle"])
+
+    sec_plain, _ = timed(run_plain, reps, 1)
+    out["plain_fp64"] = {"grid_points_per_s": P / sec_plain, "seconds": sec_plain,
+                         "launch": list(single_cfg),
+                         "form": "per request: flowmap_grid_2D + ftle_grid_2D"}
+    results = {"plain_fp64": [_lib.to_host(r["ftle"]) for r in plain]}
+    for prec in ("fp64", "fp32"):
+        b = FTLEBatch(W["flow"], reqs, [-1.0], precision=prec)
+        pc = _lib.NBX_FP64 if prec == "fp64" else _lib.NBX_FP32_DISP
+        if tune:
+            cands = [(4, blk, per) for blk in (64, 128, 256) for per in (0, 2, 4, 8)]
+            cfg, _ = autotune.tune(_lib.NBX_FLOW_SPLINE3, pc, b.run, P, reps=1,
+                                   use_cache=False, cands=cands)
+        else:
+            cfg = autotune.get(_lib.NBX_FLOW_SPLINE3, pc)
+        b.capture()
+        sec, _ = timed(b.run_graph, reps, 1)
+        results["batched_" + prec] = [r["ftle"] for r in b.results()]
+        out["batched_" + prec] = {"grid_points_per_s": P / sec, "seconds": sec,
+                                  "launch": list(cfg), "particle_steps": b.total_steps()}
+        b.close()
+    autotune.set_config(_lib.NBX_FLOW_SPLINE3, _lib.NBX_FP64, single_cfg)
+    out["speedup_batched_fp64_vs_plain"] = sec_plain / out["batched_fp64"]["seconds"]
+    out["speedup_batched_fp32_vs_plain"] = sec_plain / out["batched_fp32"]["seconds"]
+    # FTLE against the FP64 oracle, every request, sampled rows
+    S = oracle.Spline(W["grid"], _host(W["Cu"]), _host(W["Cv"]), spherical=1)
+    cores = len(os.sched_getaffinity(0))
+    per_req = {k: [] for k in results}
+    for k, q in enumerate(reqs):
+        nx = len(q["x"])
+        centre = np.arange(oracle_rows // 2, nx - 1, oracle_rows)
+        sel = np.stack([centre - 1, centre, centre + 1], 1).ravel()
+        fm_o, _, _ = oracle.flowmap_grid_2d("spline", np.array([-1.0]), q["t0"], q["T"],
+                                            q["x"][sel], q["y"], spline=S, nthreads=cores)
+        dx, dy = q["x"][1] - q["x"][0], q["y"][1] - q["y"][0]
+        f_o = oracle.ftle_grid_2d(fm_o, q["T"], dx, dy)[1::3]
+        scale = np.abs(f_o).max()
+        for name in results:
+            e = np.abs(results[name][k][centre] - f_o) / scale
+            per_req[name].append((float(e.max()), float(np.sqrt(np.mean(e ** 2))),
+                                  float(np.mean(e <= 1e-3)), float(np.mean(e <= 1e-6))))
+    err = {}
+    for name, v in per_req.items():
+        v = np.array(v)
+        err[name] = {"max_rel": float(v[:, 0].max()),
+                     "median_request_max_rel": float(np.median(v[:, 0])),
+                     "median_request_rms_rel": float(np.median(v[:, 1])),
+                     "frac_within_1e-3": float(v[:, 2].mean()),
+                     "frac_within_1e-6": float(v[:, 3].mean()),
+                     "requests_all_within_1e-3": int((v[:, 2] == 1.0).sum()),
+                     "per_request_max_rel": [round(float(a), 9) for a in v[:, 0]]}
+    out["ftle_vs_fp64_oracle"] = dict(
+        err, sample=f"every {oracle_rows}th particle row of each of the 48 requests (the oracle "
+                    "integrates the row and its two neighbours); error relative to the "
+                    "request's largest sampled oracle FTLE")
+    return out
+
+
+def cpu_baseline(W, xp, yp, fm_gpu, ftle_gpu, evec_gpu, t0, T, reps):
     """The CPU oracle (C restatement of the reference algorithm, OpenMP over
     grid rows as the reference's prange) on the full workload, on all host
-    cores this process may use; median of reps runs after one warm-up."""
+    cores this process may use; median of reps runs after one warm-up.  Also
+    the GPU result's error against it: flow map, FTLE and the p90 ridge sets
+    (grid points flagged as ridge points, and the grouped ridges)."""
     import oracle
 
     cores = len(os.sched_getaffinity(0))
@@ -262,6 +384,19 @@ def cpu_baseline(W, xp, yp, fm_gpu, ftle_gpu, t0, T, reps):
            "ftle_max_rel": float(ftle_err.max()),
            "ftle_rms_rel": float(np.sqrt(np.mean(ftle_err ** 2))),
            "ftle_frac_within_1e-6": float(np.mean(ftle_err <= 1e-6))}
+    _, flag_o = oracle.ridge_predicate(ftle, V[..., 1], xp, yp, percentile_=90)
+    _, flag_g = oracle.ridge_predicate(ftle_gpu, evec_gpu, xp, yp, percentile_=90)
+    r_o = oracle.ridges(ftle, V[..., 1], xp, yp, percentile_=90)
+    r_g = oracle.ridges(ftle_gpu, evec_gpu, xp, yp, percentile_=90)
+    same = len(r_o) == len(r_g) and all(a.shape == b.shape for a, b in zip(r_o, r_g))
+    err["ridges_p90"] = {"ridge_points_oracle": int(flag_o.sum()),
+                         "ridge_points_gpu": int(flag_g.sum()),
+                         "flagged_points_differing": int((flag_o ^ flag_g).sum()),
+                         "ridges_oracle": len(r_o), "ridges_gpu": len(r_g),
+                         "same_ridges_and_sizes": bool(same),
+                         "max_point_abs_diff_deg": float(max((np.abs(a - b).max() for a, b in
+                                                              zip(r_o, r_g)), default=0.0))
+                         if same else None}
     return cpu, err
 
 
@@ -312,61 +447,7 @@ def bench_double_gyre(args):
 
 # ---------------------------------------------------------------------------
 def bench_timeseries(args):
-    import torch
-
-    import oracle
-    from numbacs_amd import _lib, autotune
-    from numbacs_amd.batch import FTLEBatch
-    from numbacs_amd.workloads import time_series_requests
-
-    W = gridded_field(480)
-    reqs = time_series_requests()
-    out = {"workload": "FTLE time series: 48 requests (t0 = 72..119 h hourly, T cycling "
-                       "-24/-48/-72 h, seeded grids 201x101..1351x501) over a 271x101x480 "
-                       "field, one persistent launch + batched FTLE, HIP graph replay"}
-    P = sum(len(q["x"]) * len(q["y"]) for q in reqs)
-    out["particles"] = P
-    results = {}
-    for prec in ("fp64", "fp32"):
-        b = FTLEBatch(W["flow"], reqs, [-1.0], precision=prec)
-        pc = _lib.NBX_FP64 if prec == "fp64" else _lib.NBX_FP32_DISP
-        tuned, _ = autotune.tune(_lib.NBX_FLOW_SPLINE3, pc, b.run, P, reps=1, use_cache=False)
-        b.capture()
-        sec, _ = timed(b.run_graph, max(1, args.steps // 4), 1)
-        results[prec] = b.results()
-        out[prec] = {"grid_points_per_s": P / sec, "seconds": sec, "launch": tuned,
-                     "particle_steps": b.total_steps()}
-        b.close()
-    # FP32 against FP64 on every request, and FP64 against the CPU oracle on two
-    ferr, ftle_err, ftle_rms, within, split = [], [], [], [], 0
-    for r64, r32 in zip(results["fp64"], results["fp32"]):
-        d = np.abs(r32["flowmap"] - r64["flowmap"]).max(-1)
-        ferr.append(d.max())
-        # trajectories that graze the grid edge can end on either side of it
-        # (zero velocity outside): count them apart from the rounding error
-        split += int((d > 1.0).sum())
-        e = np.abs(r32["ftle"] - r64["ftle"]) / np.abs(r64["ftle"]).max()
-        ftle_err.append(e.max())
-        ftle_rms.append(np.sqrt(np.mean(e ** 2)))
-        within.append(np.mean(e <= 1e-3))
-    out["fp32_vs_fp64"] = {"flowmap_max_abs_deg": float(max(ferr)),
-                           "particles_split_at_grid_edge": split,
-                           "ftle_max_rel": float(max(ftle_err)),
-                           "ftle_median_request_max_rel": float(np.median(ftle_err)),
-                           "ftle_median_request_rms_rel": float(np.median(ftle_rms)),
-                           "ftle_frac_within_1e-3": float(np.mean(within))}
-    S = oracle.Spline(W["grid"], _host(W["Cu"]), _host(W["Cv"]), spherical=1)
-    oerr = []
-    for k in (0, 1):
-        q = reqs[k]
-        ref, _, _ = oracle.flowmap_grid_2d("spline", np.array([-1.0]), q["t0"], q["T"], q["x"],
-                                           q["y"], spline=S)
-        for prec in ("fp64", "fp32"):
-            oerr.append((prec, float(np.abs(results[prec][k]["flowmap"] - ref).max(-1).max()
-                                     / np.abs(ref).max())))
-    out["vs_fp64_oracle_first_two_requests"] = oerr
-    out["fp32_speedup"] = out["fp32"]["grid_points_per_s"] / out["fp64"]["grid_points_per_s"]
-    return out
+    return timeseries_legs(reps=max(1, args.steps // 4), tune=not args.no_tune)
 
 
 # ---------------------------------------------------------------------------
@@ -714,6 +795,49 @@ def bench_surrogate(args):
             "hybrid_ftle_max_rel_vs_plain": float(max(errs))}
 
 
+def bench_stencil(args):
+    """The FTLE stencils alone on a large flow map (8001 x 4001 points, a
+    smooth synthetic map in HBM): ftle_grid_2D (16 B read + 8 B written per
+    point) and the fused C_eig_2D + ftle_from_eig (16 B read + 24 B written),
+    timed with HIP events on the launch stream against the 8 TB/s HBM peak."""
+    import torch
+
+    from numbacs_amd import _lib
+    from numbacs_amd.diagnostics import ftle_eig_2d_device, ftle_grid_2d_device
+
+    nx, ny = 8001, 4001
+    t = _lib.torch()
+    x = t.linspace(0, 2, nx, dtype=t.float64, device=_lib.device())
+    y = t.linspace(0, 1, ny, dtype=t.float64, device=_lib.device())
+    X, Y = t.meshgrid(x, y, indexing="ij")
+    fm = t.stack([X + 0.1 * t.sin(6 * Y), Y + 0.1 * t.cos(5 * X)], -1).contiguous()
+    del X, Y
+    dx, dy = 2.0 / (nx - 1), 1.0 / (ny - 1)
+    ftle = _lib.empty((nx, ny))
+    evec = _lib.empty((nx, ny, 2))
+    out = {"workload": f"FTLE stencils on a {nx}x{ny} flow map (FP64, HBM-resident)"}
+    for name, fn, nbytes in (
+            ("ftle_grid_2d", lambda: ftle_grid_2d_device(fm, -10.0, dx, dy, out=ftle), 24),
+            ("ftle_eig_2d", lambda: ftle_eig_2d_device(fm, -10.0, dx, dy, ftle=ftle, evec=evec),
+             40)):
+        for _ in range(3):
+            fn()
+        ts = []
+        for _ in range(max(5, args.steps)):
+            e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            t.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ms = float(np.median(ts))
+        gbs = nbytes * nx * ny / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "algorithmic_bytes": nbytes * nx * ny,
+                     "bytes_per_point": nbytes, "achieved_GBs": gbs,
+                     "hbm_frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
 def bench_copycal(args):
     """A plain device copy of known size (512 MiB read + 512 MiB written,
     three dispatches): the reference for scaling rocprofv3's FETCH_SIZE and
@@ -741,7 +865,8 @@ def main():
     fn = {"gridded": bench_gridded, "double_gyre": bench_double_gyre,
           "timeseries": bench_timeseries, "precision": bench_precision,
           "surrogate": bench_surrogate, "composition": bench_composition,
-          "next": bench_next, "copycal": bench_copycal}[args.workload]
+          "next": bench_next, "copycal": bench_copycal,
+          "stencil": bench_stencil}[args.workload]
     print(json.dumps(fn(args)))
 
 

@@ -12,12 +12,13 @@
  *     a call returns once its work is enqueued unless it says otherwise.
  *   - Return value 0 = success, < 0 = error; nbx_last_error() then returns a
  *     thread-local message.
- *   - The percentile, ridge, label and ordered-ridge entry points share a
- *     grow-only device scratch buffer per device, held until the process
- *     ends: their calls must not overlap in time on one device (different
- *     host threads or streams race on it).  The flow-map, FTLE and
- *     Cauchy-Green entry points use only their arguments (and, for the
- *     persistent launch shape, a queue counter allocated per call).
+ *   - The percentile, ridge, label and ordered-ridge entry points use a
+ *     grow-only device workspace per (device, stream), held until the
+ *     process ends and locked for the duration of a call: calls on
+ *     different streams run concurrently, calls on one stream from several
+ *     host threads take turns.  The flow-map, FTLE and Cauchy-Green entry
+ *     points use only their arguments (and, for the persistent launch
+ *     shape, a queue counter allocated per call).
  *
  * The reference's own FFI for this path is numbalsoda's
  *   dop853(funcptr, u0, t_eval, rtol, atol, data=params)
@@ -347,6 +348,31 @@ int nbx_mlp_adam(float* w, const float* g, float* m, float* v, uint16_t* w16, in
                  float lr, float b1, float b2, float eps, int step, float weight_decay,
                  nbx_stream stream);
 int nbx_mlp_to_bf16(const float* x, int64_t n, uint16_t* y, nbx_stream stream);
+
+/* The trained network for fused inference: feature shift/scale (4 each) and
+ * frequencies Bf (4 x half), hidden layers as transposed bf16 weights WT[l]
+ * (width x K_l, k contiguous; K_0 = 2*half, else width) with fp32 biases,
+ * the head Wout (width x 2, bf16) with bias bout and the output scaling
+ * out_scale (2).  All device pointers. */
+typedef struct nbx_mlp_net {
+    const float* shift;
+    const float* scale;
+    const float* Bf;
+    int half, width, layers;
+    const uint16_t* WT[4];
+    const float* b[4];
+    const uint16_t* Wout;
+    const float* bout;
+    const float* out_scale;
+} nbx_mlp_net;
+
+/* out (n x 2) = displacement (physical units) for inputs inp (n x 4) =
+ * (x0, y0, t0, T): features, hidden layers and head in one launch, equal to
+ * the unfused forward pass (nbx_mlp_features, nbx_mlp_gemm mode 0,
+ * nbx_mlp_out_forward) times out_scale.  Needs half = 64, width = 256,
+ * layers = 4. */
+int nbx_mlp_infer(const nbx_mlp_net* net, const float* inp, int64_t n, float* out,
+                  nbx_stream stream);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,7 @@ SIGNATURES = {
     "nbx_mlp_adam": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, ctypes.c_float, ctypes.c_float,
                             ctypes.c_float, ctypes.c_float, _c_i, ctypes.c_float, _c_p]),
     "nbx_mlp_to_bf16": (_c_i, [_c_p, _c_i64, _c_p, _c_p]),
+    "nbx_mlp_infer": (_c_i, [_c_p, _c_p, _c_i64, _c_p, _c_p]),
     "nbx_ftle_grid_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_d, _c_p, _c_p, _c_p]),
     "nbx_C_eig_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_p, _c_p, _c_p, _c_p]),
     "nbx_C_tensor_aux_2d": (_c_i, [_c_p, _c_i, _c_i, _c_i, _c_d, _c_p, _c_p, _c_p]),
@@ -91,6 +92,13 @@ class Request(ctypes.Structure):
     _fields_ = [("t0", _c_d), ("T", _c_d), ("x", _c_p), ("nx", _c_i), ("y", _c_p), ("ny", _c_i),
                 ("mask", _c_p), ("flowmap", _c_p), ("status", _c_p), ("nsteps", _c_p),
                 ("ftle", _c_p), ("eigvec_max", _c_p), ("dx", _c_d), ("dy", _c_d)]
+
+
+class MlpNet(ctypes.Structure):
+    """struct nbx_mlp_net (include/numbacs_amd.h)."""
+    _fields_ = [("shift", _c_p), ("scale", _c_p), ("Bf", _c_p), ("half", _c_i), ("width", _c_i),
+                ("layers", _c_i), ("WT", _c_p * 4), ("b", _c_p * 4), ("Wout", _c_p),
+                ("bout", _c_p), ("out_scale", _c_p)]
 
 
 _lib = None

@@ -47,9 +47,10 @@ def _load_cache():
         return {}
 
 
-def tune(kind, precision, run, size, reps=3, persistent=True, use_cache=True):
-    """run() enqueues one flow map with the installed launch shape.  Returns
-    (best_cfg, {cfg: ms})."""
+def tune(kind, precision, run, size, reps=3, persistent=True, use_cache=True, cands=None):
+    """run() enqueues one flow map with the installed launch shape; cands
+    (optional) restricts the candidate shapes.  Returns (best_cfg,
+    {cfg: ms})."""
     t = _lib.torch()
     key = _key(kind, precision, size)
     cache = _load_cache() if use_cache else {}
@@ -59,7 +60,7 @@ def tune(kind, precision, run, size, reps=3, persistent=True, use_cache=True):
         return best, cache[key]["times"]
     old = get(kind, precision)
     times = {}
-    for cfg in candidates(kind, persistent):
+    for cfg in (cands if cands is not None else candidates(kind, persistent)):
         set_config(kind, precision, cfg)
         run()
         t.cuda.synchronize()

@@ -1,7 +1,11 @@
 // Batched FTLE time series: many flow-map requests (each its own window
-// t0, T and particle grid) over one flow, integrated by one persistent
-// work-queue launch, followed by one batched FTLE launch over every request's
-// grid.  The pair can be captured once into a HIP graph and replayed.
+// t0, T and particle grid) over one flow, integrated by one launch -- the
+// persistent work queue, or one non-persistent launch over every request's
+// particle tiles (the launch configuration's persistent occupancy 0; the
+// autotuner picks the faster) -- followed by one batched FTLE launch over
+// every request's grid.  The pair can be captured once into a HIP graph and
+// replayed.  Requests run longest window first (the tile launch dispatches
+// blocks in order, so the longest solves do not form the launch's tail).
 //
 // The reference's analogue is a Python loop over t0 calling flowmap_grid_2D
 // and ftle_grid_2D per window (examples/time_series/plot_dg_time_series.py:
@@ -68,6 +72,8 @@ struct nbx_batch {
     int ntiles = 0;
     nbx::ReqDev* d_table = nullptr;
     int64_t* d_offsets = nullptr;
+    int64_t total_slots = 0;
+    int64_t* d_slot_off = nullptr;  // tile launch: first slot of each request
     unsigned long long* d_counter = nullptr;
     nbx::FtleReqDev* d_ftle = nullptr;
     nbx::TileDev* d_tiles = nullptr;
@@ -80,10 +86,17 @@ namespace {
 
 template <class Flow, class R, int G>
 int launch_batch_queue(nbx_batch* b, const Flow& f, const LaunchCfg& cfg, hipStream_t s) {
+    const int block = cfg.block;
+    if (cfg.persistent == 0) {
+        if (b->total_slots == 0) return 0;
+        k_flowmap_tiles<Flow, R, G><<<blocks_for(b->total_slots * G, block), block, 0, s>>>(
+            f, b->d_table, b->d_slot_off, b->nreq, b->total_slots, b->rtol, b->atol, b->nmax);
+        NBX_LAUNCHED("flowmap_tiles (batch)");
+        return 0;
+    }
     const int cus = device_cu_count();
     NBX_REQUIRE(cus > 0, "cannot query the compute-unit count");
-    const int block = cfg.block;
-    const int per_cu = cfg.persistent > 0 ? cfg.persistent : 8;
+    const int per_cu = cfg.persistent;
     const int64_t want = (b->total * G + block - 1) / block;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(want, static_cast<int64_t>(cus) * per_cu));
     k_flowmap_queue<Flow, R, G><<<static_cast<unsigned>(grid), block, 0, s>>>(
@@ -136,6 +149,7 @@ void free_batch(nbx_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_table) (void)hipFree(b->d_table);
     if (b->d_offsets) (void)hipFree(b->d_offsets);
+    if (b->d_slot_off) (void)hipFree(b->d_slot_off);
     if (b->d_counter) (void)hipFree(b->d_counter);
     if (b->d_ftle) (void)hipFree(b->d_ftle);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
@@ -164,11 +178,17 @@ int nbx_batch_create(const nbx_flow* flow, const nbx_request* reqs, int nreq,
     NBX_REQUIRE(flow->kind != NBX_FLOW_BICKLEY_JET || nparams >= 12,
                 "bickley_jet needs 12 parameters");
     std::vector<ReqDev> table(nreq);
-    std::vector<int64_t> offsets(nreq + 1, 0);
+    std::vector<int64_t> offsets(nreq + 1, 0), slot_off(nreq + 1, 0);
     std::vector<FtleReqDev> ftle;
     std::vector<TileDev> tiles;
+    // longest window first (stable, so equal windows keep the caller's order)
+    std::vector<int> order(nreq);
+    for (int r = 0; r < nreq; ++r) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        return std::fabs(reqs[a].T) > std::fabs(reqs[c].T);
+    });
     for (int r = 0; r < nreq; ++r) {
-        const nbx_request& q = reqs[r];
+        const nbx_request& q = reqs[order[r]];
         NBX_REQUIRE(q.nx >= 0 && q.ny >= 0, "negative grid size in a request");
         const int64_t P = static_cast<int64_t>(q.nx) * q.ny;
         NBX_REQUIRE(P == 0 || (q.x && q.y && q.flowmap), "NULL array in a request");
@@ -176,6 +196,7 @@ int nbx_batch_create(const nbx_flow* flow, const nbx_request* reqs, int nreq,
                           q.ny > 0 ? q.ny : 1, q.mask, reinterpret_cast<double2*>(q.flowmap),
                           q.status, q.nsteps};
         offsets[r + 1] = offsets[r] + P;
+        slot_off[r + 1] = slot_off[r] + (P > 0 ? tile_slots(q.nx, q.ny) : 0);
         if (q.ftle != nullptr && P > 0) {
             const int id = static_cast<int>(ftle.size());
             ftle.push_back(FtleReqDev{reinterpret_cast<const double2*>(q.flowmap), q.ftle,
@@ -194,9 +215,11 @@ int nbx_batch_create(const nbx_flow* flow, const nbx_request* reqs, int nreq,
     b->precision = precision;
     b->nreq = nreq;
     b->total = offsets[nreq];
+    b->total_slots = slot_off[nreq];
     b->ntiles = static_cast<int>(tiles.size());
     bool ok = hipMalloc(&b->d_table, sizeof(ReqDev) * nreq) == hipSuccess &&
               hipMalloc(&b->d_offsets, sizeof(int64_t) * (nreq + 1)) == hipSuccess &&
+              hipMalloc(&b->d_slot_off, sizeof(int64_t) * (nreq + 1)) == hipSuccess &&
               hipMalloc(&b->d_counter, sizeof(unsigned long long)) == hipSuccess;
     if (ok && !ftle.empty())
         ok = hipMalloc(&b->d_ftle, sizeof(FtleReqDev) * ftle.size()) == hipSuccess &&
@@ -205,6 +228,8 @@ int nbx_batch_create(const nbx_flow* flow, const nbx_request* reqs, int nreq,
         ok = hipMemcpy(b->d_table, table.data(), sizeof(ReqDev) * nreq, hipMemcpyHostToDevice) ==
                  hipSuccess &&
              hipMemcpy(b->d_offsets, offsets.data(), sizeof(int64_t) * (nreq + 1),
+                       hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(b->d_slot_off, slot_off.data(), sizeof(int64_t) * (nreq + 1),
                        hipMemcpyHostToDevice) == hipSuccess;
     if (ok && !ftle.empty())
         ok = hipMemcpy(b->d_ftle, ftle.data(), sizeof(FtleReqDev) * ftle.size(),

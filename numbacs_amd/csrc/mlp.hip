@@ -278,6 +278,122 @@ __global__ void k_to_bf16(const float* __restrict__ x, int64_t n, uint16_t* __re
     if (i < n) y[i] = f2bf(x[i]);
 }
 
+// ---------------------------------------------------------------------------
+// Fused inference: Fourier features, the four hidden layers and the head in
+// one launch.  A 256-thread block takes 64 particles; their activations stay
+// in LDS (two 64 x 256 bf16 buffers, rows padded to 264 elements), each
+// wavefront computes a 64 x 64 quarter of a layer's output as 4 x 4 MFMA
+// 16x16x32 tiles, reading its weight fragments straight from global memory
+// (weights stored transposed, k contiguous, so a lane's 8 k-values are one
+// 16-byte load; the next k-step's fragments are loaded before the current
+// step's MFMAs).  Every step is the unfused path's arithmetic in its order:
+// the features of k_features, per layer the same k-steps of 32 accumulated
+// in order then bias + GELU + bf16 (k_gemm's epilogue), the head's
+// sequential fp32 sums of k_out_fwd and the output scaling -- so the result
+// equals the unfused forward pass bit for bit.
+constexpr int kInferRows = 64, kInferWidth = 256, kInferHalf = 64, kActLd = kInferWidth + 8;
+
+struct MlpNetDev {
+    const float* shift;
+    const float* scale;
+    const float* Bf;
+    const uint16_t* WT[4];
+    const float* b[4];
+    const uint16_t* Wout;
+    const float* bout;
+    const float* out_scale;
+};
+
+__global__ __launch_bounds__(256) void k_mlp_infer(MlpNetDev net, const float* __restrict__ inp,
+                                                   int64_t n, float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint16_t act[2][kInferRows][kActLd];
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kInferRows;
+    // ---- Fourier features (k_features): thread -> row t/4, 16 frequencies
+    {
+        const int r = t >> 2, c0 = (t & 3) * 16;
+        const int64_t gr = r0 + r < n ? r0 + r : n - 1;
+        float u[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) u[d] = (inp[4 * gr + d] - net.shift[d]) / net.scale[d];
+#pragma unroll 4
+        for (int i = 0; i < 16; ++i) {
+            const int c = c0 + i;
+            float z = 0.f;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) z += u[d] * net.Bf[d * kInferHalf + c];
+            z *= 6.283185307179586f;
+            act[0][r][c] = f2bf(sinf(z));
+            act[0][r][kInferHalf + c] = f2bf(cosf(z));
+        }
+    }
+    __syncthreads();
+    // ---- hidden layers
+    const int r16 = lane & 15, kk = 8 * (lane >> 4);
+    int cur = 0;
+#pragma unroll 1
+    for (int l = 0; l < 4; ++l) {
+        const int K = l == 0 ? 2 * kInferHalf : kInferWidth;
+        const uint16_t* __restrict__ WT = net.WT[l];
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        bf16x8 bnext[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bnext[j] = *reinterpret_cast<const bf16x8*>(
+                WT + static_cast<long>(64 * w + 16 * j + r16) * K + kk);
+#pragma unroll 1
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            bf16x8 bcur[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bcur[j] = bnext[j];
+            if (k0 + 32 < K) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    bnext[j] = *reinterpret_cast<const bf16x8*>(
+                        WT + static_cast<long>(64 * w + 16 * j + r16) * K + k0 + 32 + kk);
+            }
+            bf16x8 af[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                af[i] = *reinterpret_cast<const bf16x8*>(&act[cur][16 * i + r16][k0 + kk]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bcur[j], acc[i][j],
+                                                                        0, 0, 0);
+        }
+        const int nxt = cur ^ 1;
+        const int rq = 4 * (lane >> 4);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int nc = 64 * w + 16 * j + r16;
+            const float bias = net.b[l][nc];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    act[nxt][16 * i + rq + q][nc] = f2bf(gelu(acc[i][j][q] + bias));
+        }
+        __syncthreads();
+        cur = nxt;
+    }
+    // ---- head (k_out_fwd's sequential sums) and output scaling
+    if (t < 2 * kInferRows) {
+        const int r = t >> 1, c = t & 1;
+        if (r0 + r < n) {
+            float sum = 0.f;
+            for (int k = 0; k < kInferWidth; ++k) sum += bf2f(act[cur][r][k]) * bf2f(net.Wout[2 * k + c]);
+            out[2 * (r0 + r) + c] = (sum + net.bout[c]) * net.out_scale[c];
+        }
+    }
+}
+
 int check_gemm_shape(int M, int N, int K) {
     NBX_REQUIRE(M > 0 && N > 0 && K > 0, "empty GEMM");
     NBX_REQUIRE(M % BM == 0 && N % BN == 0 && K % BK == 0,
@@ -388,6 +504,28 @@ int nbx_mlp_adam(float* w, const float* g, float* m, float* v, uint16_t* w16, in
     k_adam<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(w, g, m, v, w16, n, lr, b1, b2, eps,
                                                               c1, c2, weight_decay);
     NBX_LAUNCHED("mlp_adam");
+    return 0;
+}
+
+int nbx_mlp_infer(const nbx_mlp_net* net, const float* inp, int64_t n, float* out,
+                  nbx_stream stream) {
+    NBX_REQUIRE(net != nullptr, "NULL network");
+    NBX_REQUIRE(n >= 0, "negative particle count");
+    NBX_REQUIRE(net->half == kInferHalf && net->width == kInferWidth && net->layers == 4,
+                "the fused inference kernel runs the 128-feature, 4 x 256 network");
+    if (n == 0) return 0;
+    NBX_REQUIRE(inp != nullptr && out != nullptr, "NULL array");
+    MlpNetDev d{net->shift, net->scale, net->Bf, {}, {}, net->Wout, net->bout, net->out_scale};
+    for (int l = 0; l < 4; ++l) {
+        NBX_REQUIRE(net->WT[l] != nullptr && net->b[l] != nullptr, "NULL layer");
+        d.WT[l] = net->WT[l];
+        d.b[l] = net->b[l];
+    }
+    NBX_REQUIRE(d.shift && d.scale && d.Bf && d.Wout && d.bout && d.out_scale, "NULL parameter");
+    const int64_t blocks = (n + kInferRows - 1) / kInferRows;
+    NBX_REQUIRE(blocks < (static_cast<int64_t>(1) << 31), "too many particles for one launch");
+    k_mlp_infer<<<static_cast<unsigned>(blocks), 256, 0, as_stream(stream)>>>(d, inp, n, out);
+    NBX_LAUNCHED("mlp_infer");
     return 0;
 }
 

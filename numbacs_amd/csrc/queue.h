@@ -143,4 +143,48 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap_queue(
     }
 }
 
+// One non-persistent launch over a table of requests in the grid launch's
+// tile order: request r owns the launch's slots [slot_off[r], slot_off[r+1])
+// (tile_slots(nx, ny) of them, a multiple of 64, so every wavefront lies in
+// one request) and maps them to its particles with tile_particle.  The
+// batch's alternative to the work queue when its particles' spatial
+// locality is worth more than the queue's refills (the autotuner times
+// both: persistent occupancy 0 selects this kernel).
+template <class Flow, class R, int G>
+__global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap_tiles(
+    Flow f, const ReqDev* __restrict__ table, const int64_t* __restrict__ slot_off, int nreq,
+    int64_t total_slots, double rtol, double atol, long long nmax) {
+    const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const int64_t q = tid / G;
+    if (q >= total_slots) return;  // whole groups leave together
+    int lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (slot_off[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    const ReqDev& Rq = table[lo];
+    const int64_t p = tile_particle(q - slot_off[lo], Rq.nx, Rq.ny);
+    if (p < 0) return;
+    const bool lead = (tid & (G - 1)) == 0;
+    if (Rq.mask != nullptr && Rq.mask[p]) {
+        if (lead) {
+            Rq.out[p] = make_double2(0.0, 0.0);
+            if (Rq.status) Rq.status[p] = 0;
+            if (Rq.nsteps) Rq.nsteps[p] = 0;
+        }
+        return;
+    }
+    const int64_t i = p / Rq.ny;
+    Dop853State<R> s;
+    dop853_init(f, s, Rq.s0, Rq.s1, Rq.xs[i], Rq.ys[p - i * Rq.ny], rtol, atol);
+    while (dop853_step(f, s, rtol, atol, nmax)) {
+    }
+    if (lead) {
+        Rq.out[p] = make_double2(s.px(), s.py());
+        if (Rq.status) Rq.status[p] = s.idid;
+        if (Rq.nsteps) Rq.nsteps[p] = s.nstep;
+    }
+}
+
 }  // namespace nbx

@@ -13,6 +13,7 @@ Network (defaults): 128 Fourier features (64 frequencies), 4 hidden layers
 of 256 with tanh-form GELU, a 2-output linear head.  The numpy restatement
 used to check the kernels' gradients is oracle/mlp.py.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -20,6 +21,8 @@ import numpy as np
 from . import _lib
 
 ROWS = 64  # GEMM row granularity (batches are padded to a multiple)
+# probe particles of a 64x64 tile (offsets inside the tile, clipped to the grid)
+PROBES = ((8, 8), (8, 55), (55, 8), (55, 55), (32, 32))
 
 
 def _pad(n):
@@ -135,8 +138,41 @@ class MLPSurrogate:
                                       weight_decay, s))
         return loss
 
-    def predict_device(self, inp, chunk=1 << 20):
-        """Displacement (n, 2) float32 (physical units) for inputs (n, 4)."""
+    def _inference_net(self):
+        """The ctypes description of the current weights for the fused
+        inference kernel (hidden weights transposed, k contiguous), rebuilt
+        when training has moved the weights."""
+        if getattr(self, "_infer_step", None) != self.step:
+            self._WT = [w.t().contiguous() for w in self.W16[:-1]]
+            net = _lib.MlpNet()
+            net.shift, net.scale, net.Bf = (_lib.ptr(self.shift), _lib.ptr(self.scale),
+                                            _lib.ptr(self.B))
+            net.half, net.width, net.layers = self.F // 2, self.width, self.layers
+            for l in range(self.layers):
+                net.WT[l] = _lib.ptr(self._WT[l])
+                net.b[l] = _lib.ptr(self.b[l])
+            net.Wout, net.bout = _lib.ptr(self.W16[-1]), _lib.ptr(self.b[-1])
+            net.out_scale = _lib.ptr(self.out_scale)
+            self._net, self._infer_step = net, self.step
+        return self._net
+
+    def predict_device(self, inp, out=None):
+        """Displacement (n, 2) float32 (physical units) for inputs (n, 4)
+        float32, by the fused inference kernel (one launch)."""
+        t = _lib.torch()
+        n = int(inp.shape[0])
+        if out is None:
+            out = t.empty((n, 2), dtype=t.float32, device=_lib.device())
+        if self.F != 128 or self.width != 256 or self.layers != 4:
+            return self.predict_device_unfused(inp)
+        _lib.check(_lib.load().nbx_mlp_infer(ctypes.byref(self._inference_net()),
+                                             _lib.ptr(inp.contiguous()), n, _lib.ptr(out),
+                                             _lib.stream_ptr()))
+        return out
+
+    def predict_device_unfused(self, inp, chunk=1 << 20):
+        """The same displacement by the layer-by-layer kernels (features,
+        one GEMM per layer, head) in chunks of up to chunk particles."""
         t = _lib.torch()
         n = int(inp.shape[0])
         out = t.empty((n, 2), dtype=t.float32, device=_lib.device())
@@ -224,9 +260,8 @@ def train(net, inputs, targets, epochs=3, batch=8192, lr=2e-3, lr_min=1e-4, seed
     return losses
 
 
-def flowmap_with_fallback(net, funcptr, params, x, y, t0, T, tol, tile=64,
-                          probes=((8, 8), (8, 55), (55, 8), (55, 55), (32, 32)), rtol=1e-6,
-                          atol=1e-8):
+def flowmap_with_fallback(net, funcptr, params, x, y, t0, T, tol, tile=64, probes=PROBES,
+                          rtol=1e-6, atol=1e-8):
     """Flow map (nx, ny, 2) device tensor from the surrogate, with per-tile
     probe particles integrated by DOP853 and every tile whose probe
     displacement error exceeds tol integrated by DOP853 in full.  Returns
@@ -261,5 +296,6 @@ def flowmap_with_fallback(net, funcptr, params, x, y, t0, T, tol, tile=64,
             fm[i0:i0 + d["nx"], j0:j0 + d["ny"]] = d["flowmap"]
         b.close()
     info = dict(tiles=len(tiles), fallback_tiles=len(bad), probes=len(pidx),
-                probe_err_max=float(err.max()), probe_err_median=float(np.median(err)))
+                probe_err_max=float(err.max()), probe_err_median=float(np.median(err)),
+                fallback=bad)
     return fm, info

@@ -170,22 +170,68 @@ def trained(lib):
     return dict(S=S, f=f, net=net, untrained=untrained, losses=losses, inp=inp, tgt=tgt)
 
 
-def test_surrogate_learns_from_integrator(lib, trained):
-    """What the trained network gains over the method without it: its
-    displacement error on held-out DOP853 samples is far below the
-    untrained network's and below a zero-displacement predictor's."""
+def test_fused_inference_equals_unfused(lib, trained):
+    """nbx_mlp_infer (features, 4 layers and head in one launch) equals the
+    layer-by-layer kernels bit for bit, including a partial last block."""
+    import torch
+
     S, f = trained["S"], trained["f"]
     box = ((-100.0, 35.0), (-5.0, 45.0))
-    reqs = S.sample_requests(8, 40, box, (30.0, 39.0), (-12.0, -12.0), seed=99)
-    inp, tgt = S.integrate_samples(f, [-1.0], reqs)
-    pred = trained["net"].predict_device(inp)
-    pred0 = trained["untrained"].predict_device(inp)
-    err = float((pred - tgt).norm(dim=1).mean())
-    err0 = float((pred0 - tgt).norm(dim=1).mean())
-    zero = float(tgt.norm(dim=1).mean())
-    assert trained["losses"][-1] < 0.2 * trained["losses"][0]
-    assert err < 0.5 * zero, (err, zero)
-    assert err < 0.5 * err0, (err, err0)
+    inp, _ = S.integrate_samples(f, [-1.0], S.sample_requests(2, 25, box, (30.0, 39.0),
+                                                              (-12.0, -12.0), seed=3))
+    inp = inp[:1001].contiguous()
+    a = trained["net"].predict_device(inp)
+    b = trained["net"].predict_device_unfused(inp)
+    assert torch.equal(a, b)
+
+
+def test_surrogate_against_integrator_same_tiles(lib, trained):
+    """What the trained network gains over the method without it, on the
+    same 64x64 tiles: the surrogate alone (one fused launch) against DOP853
+    (the integrator the hybrid falls back to).  It is much faster, and far
+    better than the untrained network and a zero-displacement predictor, but
+    its displacement error stays far above the fallback tolerance of 0.05
+    grid steps, so with that tolerance every tile falls back (the hybrid is
+    the integrator plus the probes and the inference)."""
+    import torch
+
+    from numbacs_amd.integration import flowmap_grid_2d_device
+
+    S, f, net = trained["S"], trained["f"], trained["net"]
+    x, y = np.linspace(-100, 35, 6 * 64), np.linspace(-5, 45, 3 * 64)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    xd, yd = lib.to_dev(x), lib.to_dev(y)
+    inp = S.grid_inputs(x, y, 36.0, -12.0)
+    fm = lib.empty((len(x), len(y), 2))
+    disp = lib.empty((len(x) * len(y), 2), "f32")
+
+    def med_ms(fn):
+        ts = []
+        for _ in range(6):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts[1:]))
+
+    t_int = med_ms(lambda: flowmap_grid_2d_device(f, 36.0, -12.0, xd, yd, [-1.0], out=fm))
+    t_net = med_ms(lambda: net.predict_device(inp, out=disp))
+    exact = lib.to_host(fm) - np.stack([X, Y], -1)
+    pred = lib.to_host(disp).reshape(exact.shape)
+    pred0 = lib.to_host(trained["untrained"].predict_device(inp)).reshape(exact.shape)
+    err = np.abs(pred - exact).max(-1)
+    err0 = np.abs(pred0 - exact).max(-1)
+    zero = np.abs(exact).max(-1)
+    tol = 0.05 * (x[1] - x[0])
+    print(f"integrator {t_int:.3f} ms, surrogate {t_net:.3f} ms ({t_int / t_net:.1f}x); "
+          f"median error {np.median(err):.3g} deg (untrained {np.median(err0):.3g}, zero "
+          f"{np.median(zero):.3g}), within 0.05 dx: {np.mean(err <= tol):.4f}", flush=True)
+    assert t_net * 3 < t_int
+    assert np.median(err) < 0.5 * np.median(zero)
+    assert np.median(err) < 0.5 * np.median(err0)
+    assert np.median(err) > tol  # the fallback tolerance is not met: tiles fall back
 
 
 def test_fallback_tiles_equal_integrator(lib, trained):
