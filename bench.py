@@ -815,8 +815,11 @@ def bench_stencil(args):
     dx, dy = 2.0 / (nx - 1), 1.0 / (ny - 1)
     ftle = _lib.empty((nx, ny))
     evec = _lib.empty((nx, ny, 2))
-    out = {"workload": f"FTLE stencils on a {nx}x{ny} flow map (FP64, HBM-resident)"}
+    fm2 = t.empty_like(fm)
+    out = {"workload": f"FTLE stencils on a {nx}x{ny} flow map (FP64, HBM-resident); a "
+                       "device copy of the flow map as the achievable-bandwidth reference"}
     for name, fn, nbytes in (
+            ("copy_reference", lambda: fm2.copy_(fm), 32),
             ("ftle_grid_2d", lambda: ftle_grid_2d_device(fm, -10.0, dx, dy, out=ftle), 24),
             ("ftle_eig_2d", lambda: ftle_eig_2d_device(fm, -10.0, dx, dy, ftle=ftle, evec=evec),
              40)):

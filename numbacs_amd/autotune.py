@@ -18,9 +18,15 @@ CACHE = os.path.join(os.path.expanduser("~"), ".cache", "numbacs_amd", "autotune
 
 
 def candidates(kind, persistent=True):
+    """Launch shapes: lanes per particle x threads per block x persistent
+    blocks per CU (0: one launch over the particle tiles); spline flows add
+    the LDS-staged launch (4 lanes, 64-thread blocks, persistent -1)."""
     lanes = (1, 2, 4) if kind == _lib.NBX_FLOW_SPLINE3 else (1,)
     pers = (0, 2, 4, 8) if persistent else (0,)
-    return [(g, b, p) for g in lanes for b in (64, 128, 256) for p in pers]
+    out = [(g, b, p) for g in lanes for b in (64, 128, 256) for p in pers]
+    if kind == _lib.NBX_FLOW_SPLINE3:
+        out.append((4, 64, -1))
+    return out
 
 
 def get(kind, precision):

@@ -190,11 +190,11 @@ __device__ __forceinline__ void dop853_init(const Flow& f, Dop853State<R>& s, do
     s.h = s.posneg >= 0 ? h : -h;
 }
 
-// One attempted step (accepted or rejected).  Sets s.idid != 0 when the
-// solve has ended; returns true while it is still running.
-template <class R, class Flow>
-__device__ __forceinline__ bool dop853_step(const Flow& f, Dop853State<R>& s, double rtol,
-                                            double atol, long long nmax) {
+// The head of one attempted step: the step budget and underflow checks and
+// the clipping of the last step to xend.  Sets s.idid != 0 and returns false
+// when the solve ends here; otherwise s.h is the step about to be taken.
+template <class R>
+__device__ __forceinline__ bool dop853_step_begin(Dop853State<R>& s, long long nmax) {
     using namespace dop;
     if (s.nstep > nmax) {
         s.idid = -2;
@@ -208,6 +208,15 @@ __device__ __forceinline__ bool dop853_step(const Flow& f, Dop853State<R>& s, do
         s.h = s.xend - s.x;
         s.last = true;
     }
+    return true;
+}
+
+// The stages, error estimate and step-size control of the attempted step
+// that dop853_step_begin prepared.
+template <class R, class Flow>
+__device__ __forceinline__ bool dop853_step_body(const Flow& f, Dop853State<R>& s, double rtol,
+                                                 double atol) {
+    using namespace dop;
     ++s.nstep;
     const double x = s.x, h = s.h;
     // Stage sums in double for both forms: DOP853's coefficients are large
@@ -324,6 +333,15 @@ __device__ __forceinline__ bool dop853_step(const Flow& f, Dop853State<R>& s, do
     }
     s.h = hnew;
     return true;
+}
+
+// One attempted step (accepted or rejected).  Sets s.idid != 0 when the
+// solve has ended; returns true while it is still running.
+template <class R, class Flow>
+__device__ __forceinline__ bool dop853_step(const Flow& f, Dop853State<R>& s, double rtol,
+                                            double atol, long long nmax) {
+    if (!dop853_step_begin(s, nmax)) return false;
+    return dop853_step_body(f, s, rtol, atol);
 }
 
 }  // namespace nbx

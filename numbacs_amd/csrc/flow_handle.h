@@ -81,9 +81,9 @@ Linear3<R> make_linear(const nbx_flow* fl, double p0) {
     return s;
 }
 
-template <int G, class R>
-Spline3<R, G> regroup(const Spline3<R, 1, false>& a) {
-    Spline3<R, G> s;
+template <int G, class R, bool S = false>
+Spline3<R, G, false, S> regroup(const Spline3<R, 1, false>& a) {
+    Spline3<R, G, false, S> s;
     s.C = a.C;
     for (int d = 0; d < 3; ++d) {
         s.lo[d] = a.lo[d];

@@ -312,6 +312,166 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap(Flow f, 
     }
 }
 
+// Spline coefficients staged through LDS, 4 lanes per particle, one
+// wavefront per 64-thread block (16 particles, a 4 x 4 tile of the grid).
+// Before every attempted step the wavefront bounds where its stencils will
+// fall -- times p0*[x, x+h] of its particles, positions from the current
+// point along h times the current derivative with half that displacement
+// as margin on either side, in the spline's lookup coordinates -- copies
+// that box of coefficient elements (bt x bx x by, at most kStageElems of 16
+// bytes) from HBM/L2 into LDS with row-contiguous loads, and every stencil
+// inside it is then read from LDS (Spline3<..., S = true>); stencils outside
+// (a stage that strays, a box too large) are gathered from global memory as
+// in k_flowmap.  Per step a wavefront's stencils span a few hours and a few
+// grid cells: median box 8 KiB, 23 KiB at the largest step of a typical
+// tile on the gridded workload, against 192 KiB of gathered coefficients.
+// The integrator's arithmetic is that of k_flowmap, so results are equal.
+constexpr int kStageElems = 1280;  // 20 KiB: 8 one-wave blocks per CU
+
+// One global_load_lds_dwordx4: lane l's 16 bytes at src go to LDS at
+// dst + 16 l (dst wavefront-uniform).  The builtin exists only in the device
+// compilation pass.
+__device__ __forceinline__ void load_lds16(const void* src, void* dst) {
+#if __HIP_DEVICE_COMPILE__
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0,
+                                     0);
+#endif
+}
+
+__device__ __forceinline__ double wave_min_groups(double v) {
+    // the four lanes of a group hold equal values: reduce over groups
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ double wave_max_groups(double v) {
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+template <class R>
+__global__ __launch_bounds__(64) void k_flowmap_staged(
+    Spline3<R, 4, false, true> f, GridSource src, int64_t P, double s0, double s1, double rtol,
+    double atol, const uint8_t* __restrict__ mask, long long nmax, double2* __restrict__ out,
+    int32_t* __restrict__ status, int32_t* __restrict__ nsteps) {
+    using Pt = typename CoefPair<R>::type;
+    __shared__ Pt stage[kStageElems];
+    const int lane = static_cast<int>(threadIdx.x);
+    const int64_t slot = static_cast<int64_t>(blockIdx.x) * 16 + (lane >> 2);
+    const int64_t p = slot < P ? src.particle(slot) : -1;
+    const bool lead = (lane & 3) == 0;
+    bool running = p >= 0;
+    if (running && mask != nullptr && mask[p]) {
+        if (lead) {
+            out[p] = make_double2(0.0, 0.0);
+            if (status) status[p] = NBX_ST_MASKED;
+            if (nsteps) nsteps[p] = 0;
+        }
+        running = false;
+    }
+    f.box.lds = stage;
+    f.box.on = false;
+    Dop853State<R> s;
+    if (running) {
+        double px, py;
+        src.pos(p, px, py);
+        dop853_init(f, s, s0, s1, px, py, rtol, atol);
+    }
+    auto finish = [&]() {
+        if (lead) {
+            out[p] = make_double2(s.px(), s.py());
+            if (status) status[p] = s.idid;
+            if (nsteps) nsteps[p] = s.nstep;
+        }
+    };
+    const double inf = __builtin_inf();
+    for (;;) {
+        bool stepping = false;
+        if (running) {
+            stepping = dop853_step_begin(s, nmax);
+            if (!stepping) {
+                finish();
+                running = false;
+            }
+        }
+        if (!__any(stepping)) break;
+        // ---- the step's box (wavefront-uniform)
+        double tlo = inf, thi = -inf, xlo = inf, xhi = -inf, ylo = inf, yhi = -inf;
+        if (stepping) {
+            const double t0 = f.p0 * s.x, t1 = f.p0 * (s.x + s.h);
+            tlo = fmin(t0, t1);
+            thi = fmax(t0, t1);
+            const double ex = s.h * static_cast<double>(s.k1x);
+            const double ey = s.h * static_cast<double>(s.k1y);
+            const double m = 0.5 * (fabs(ex) + fabs(ey));
+            const double px = s.px(), py = s.py();
+            const double a = f.lookup_x(px + fmin(0.0, ex) - m);
+            const double b = f.lookup_x(px + fmax(0.0, ex) + m);
+            if (a <= b) {
+                xlo = a;
+                xhi = b;
+            } else {  // the box straddles the longitude wrap: no staging
+                xlo = -inf;
+                xhi = inf;
+            }
+            ylo = py + fmin(0.0, ey) - m;
+            yhi = py + fmax(0.0, ey) + m;
+        }
+        tlo = wave_min_groups(tlo);
+        thi = wave_max_groups(thi);
+        xlo = wave_min_groups(xlo);
+        xhi = wave_max_groups(xhi);
+        ylo = wave_min_groups(ylo);
+        yhi = wave_max_groups(yhi);
+        int c0[3], c1[3];
+        const double q0[3] = {tlo, xlo, ylo}, q1[3] = {thi, xhi, yhi};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            R fr;
+            f.cell(d, fmin(fmax(q0[d], f.lo[d]), f.hi[d]), c0[d], fr);
+            f.cell(d, fmin(fmax(q1[d], f.lo[d]), f.hi[d]), c1[d], fr);
+        }
+        const int bt = c1[0] - c0[0] + 4, bx = c1[1] - c0[1] + 4, by = c1[2] - c0[2] + 4;
+        const int size = bt * bx * by;
+        const bool fits = xlo > -inf && by <= 64 && ((size + 63) & ~63) <= kStageElems;
+        f.box.it0 = c0[0];
+        f.box.ix0 = c0[1];
+        f.box.iy0 = c0[2];
+        f.box.bt = bt;
+        f.box.bx = bx;
+        f.box.by = by;
+        f.box.on = fits;
+        if (fits) {
+            // element e = (a*bx + b)*by + c of the box, 64 a pass: each
+            // global_load_lds_dwordx4 writes the wavefront's 64 elements to
+            // consecutive LDS slots, the loads all in flight at once.  Lanes
+            // past the end (last pass) reload the last element into slots
+            // below the rounded-up size, which no stencil reads.
+            const float rby = __builtin_amdgcn_rcpf(static_cast<float>(by));
+            const float rbx = __builtin_amdgcn_rcpf(static_cast<float>(bx));
+            const int passes = (size + 63) >> 6;
+            for (int k = 0; k < passes; ++k) {
+                const int e = min(k * 64 + lane, size - 1);
+                const int r = static_cast<int>((static_cast<float>(e) + 0.5f) * rby);
+                const int c = e - r * by;
+                const int a = static_cast<int>((static_cast<float>(r) + 0.5f) * rbx);
+                const int b = r - a * bx;
+                const Pt* src_e = f.C + (static_cast<long>(c0[0] + a) * f.st +
+                                         static_cast<long>(c0[1] + b) * f.sx + (c0[2] + c));
+                load_lds16(src_e, stage + k * 64);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        __syncthreads();
+        if (stepping) {
+            running = dop853_step_body(f, s, rtol, atol);
+            if (!running) finish();
+        }
+        __syncthreads();  // every read of the box is done before the next copy
+    }
+}
+
 template <class Flow, class R>
 __global__ void k_flow_eval(Flow f, const double* __restrict__ t, const double2* __restrict__ pts,
                             int64_t n, double2* __restrict__ out) {
@@ -409,6 +569,17 @@ int dispatch_solve(const nbx_flow* fl, const double* params, int np, const Src& 
             return launch_solve<Spline3<R, 1, true>, R, 1>(make_spline<R, true>(fl, params[0]),
                                                            src, a, cfg, s);
         const Spline3<R, 1> sp = make_spline<R, false>(fl, params[0]);
+        if constexpr (std::is_same<Src, GridSource>::value) {
+            if (cfg.persistent < 0) {  // LDS-staged coefficients, 4 lanes, one-wave blocks
+                const int64_t slots = src.slots();
+                if (slots == 0) return 0;
+                k_flowmap_staged<R><<<blocks_for(slots * 4, 64), 64, 0, s>>>(
+                    regroup<4, R, true>(sp), src, slots, a.s0, a.s1, a.rtol, a.atol, a.mask,
+                    a.nmax, a.out, a.status, a.nsteps);
+                NBX_LAUNCHED("flowmap_staged");
+                return 0;
+            }
+        }
         if (cfg.lanes == 4) return launch_solve<Spline3<R, 4>, R, 4>(regroup<4>(sp), src, a, cfg, s);
         if (cfg.lanes == 2) return launch_solve<Spline3<R, 2>, R, 2>(regroup<2>(sp), src, a, cfg, s);
         return launch_solve<Spline3<R, 1>, R, 1>(sp, src, a, cfg, s);
@@ -675,8 +846,11 @@ int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int b
                 "block_threads must be 64, 128 or 256");
     NBX_REQUIRE(kind == NBX_FLOW_SPLINE3 || lanes_per_particle == 1,
                 "analytic flows run one lane per particle");
-    NBX_REQUIRE(persistent_blocks_per_cu >= 0 && persistent_blocks_per_cu <= 64,
-                "persistent_blocks_per_cu must be in [0, 64]");
+    NBX_REQUIRE(persistent_blocks_per_cu >= -1 && persistent_blocks_per_cu <= 64,
+                "persistent_blocks_per_cu must be in [-1, 64]");
+    NBX_REQUIRE(persistent_blocks_per_cu != -1 ||
+                    (kind == NBX_FLOW_SPLINE3 && lanes_per_particle == 4 && block_threads == 64),
+                "the LDS-staged launch (-1) is for spline flows at 4 lanes, 64-thread blocks");
     std::lock_guard<std::mutex> lock(g_cfg_mutex);
     g_cfg[kind][precision] = LaunchCfg{lanes_per_particle, block_threads, persistent_blocks_per_cu};
     return 0;

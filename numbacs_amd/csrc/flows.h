@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 namespace nbx {
 
@@ -272,10 +273,29 @@ __device__ __forceinline__ double group_sum(double v) {
 // kernels built without it (L = false) serve modes 0 and 2 with fewer
 // registers (FP32 4-lane flow map: 167 instead of 234 VGPRs, 3 waves per
 // SIMD instead of 2; FP64: 221 instead of 256).
-template <class R, int G = 1, bool L = false>
+// A box of coefficient elements copied to LDS by a one-wavefront block for
+// the stencils of one DOP853 step (cells it0.., ix0.., iy0.. of the
+// coefficient array, bt x bx x by elements, y fastest): a stencil whose 4 x 4
+// x 4 elements all lie inside is read from LDS, any other from HBM/L2.
+template <class P>
+struct StageBox {
+    const P* lds = nullptr;
+    int it0 = 0, ix0 = 0, iy0 = 0, bt = 0, bx = 0, by = 0;
+    bool on = false;
+    __device__ __forceinline__ bool covers(int it, int ix, int iy) const {
+        return on && static_cast<unsigned>(it - it0) <= static_cast<unsigned>(bt - 4) &&
+               static_cast<unsigned>(ix - ix0) <= static_cast<unsigned>(bx - 4) &&
+               static_cast<unsigned>(iy - iy0) <= static_cast<unsigned>(by - 4);
+    }
+};
+struct NoStageBox {};
+
+template <class R, int G = 1, bool L = false, bool S = false>
 struct Spline3 {
     using P = typename CoefPair<R>::type;
     const P* __restrict__ C;
+    // S (4-lane form only): stencils inside the step's LDS box are read there
+    std::conditional_t<S, StageBox<P>, NoStageBox> box;
     double lo[3], hi[3], dinv[3];
     int nm2[3];   // n - 2 per axis: the largest cell index
     int sx, st;   // strides (in pairs) of the x and t axes
@@ -321,6 +341,35 @@ struct Spline3 {
         }
     }
 
+    // sum over the 4 x 4 (t, x) rows of one y-column starting at lbase (time
+    // and x strides ts, xs in elements) weighted by wt, wx -- the 4-lane
+    // gather of one lane, from HBM/L2 or from the LDS box
+    __device__ __forceinline__ static void column_sums(const P* lbase, int ts, int xs,
+                                                       const R wt[4], const R wx[4], R& au,
+                                                       R& av) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            R ru = 0, rv = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                R cu, cv;
+                if constexpr (sizeof(R) == sizeof(double)) {
+                    const P e = lbase[a * ts + b * xs];
+                    cu = e.x;
+                    cv = e.y;
+                } else {
+                    const P e = lbase[(a & ~1) * ts + b * xs];
+                    cu = (a & 1) ? e.z : e.x;
+                    cv = (a & 1) ? e.w : e.y;
+                }
+                ru = fma(wx[b], cu, ru);
+                rv = fma(wx[b], cv, rv);
+            }
+            au = fma(wt[a], ru, au);
+            av = fma(wt[a], rv, av);
+        }
+    }
+
     // (u, v) at an in-box point
     __device__ __forceinline__ void eval_in(double qt, double qx, double qy, R& su,
                                             R& sv) const {
@@ -354,19 +403,16 @@ struct Spline3 {
             // the group's four lanes read each 4-pair row in one load; the
             // y-weight is applied last, once per lane
             const int g = static_cast<int>(__lane_id() & 3);
-            const P* lbase = base + g;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                R ru = 0, rv = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    R cu, cv;
-                    coef(lbase, a, b, 0, cu, cv);
-                    ru = fma(wx[b], cu, ru);
-                    rv = fma(wx[b], cv, rv);
+            bool staged = false;
+            if constexpr (S) staged = box.covers(it, ix, iy);
+            if (staged) {
+                if constexpr (S) {
+                    const P* lb = box.lds + ((it - box.it0) * box.bx + (ix - box.ix0)) * box.by +
+                                  (iy - box.iy0) + g;
+                    column_sums(lb, box.bx * box.by, box.by, wt, wx, au, av);
                 }
-                au = fma(wt[a], ru, au);
-                av = fma(wt[a], rv, av);
+            } else {
+                column_sums(base + g, st, sx, wt, wx, au, av);
             }
             const R wc = pick4(wy, g);
             au *= wc;
@@ -469,13 +515,19 @@ struct Spline3 {
         }
     }
 
+    // the x lookup coordinate of a position: the longitude wrapped into
+    // [-180, 180) (spherical=1) or [0, 360) (spherical=2), else itself
+    __device__ __forceinline__ double lookup_x(double y0) const {
+        if (spherical == 1) return py_mod(y0 - 180, 360) - 180;
+        if (spherical != 0) return py_mod(y0, 360);
+        return y0;
+    }
+
     __device__ __forceinline__ void operator()(double t, double y0, double y1, R& f0,
                                                R& f1) const {
         const double tt = p0 * t;
         // one inlined spline evaluation per call for every coordinate mode
-        double xx = y0;
-        if (spherical == 1) xx = py_mod(y0 - 180, 360) - 180;
-        else if (spherical != 0) xx = py_mod(y0, 360);
+        const double xx = lookup_x(y0);
         R su, sv;
         eval(tt, xx, y1, su, sv);
         if (spherical == 0) {

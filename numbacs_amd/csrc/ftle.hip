@@ -19,26 +19,68 @@
 
 namespace nbx {
 
-__global__ __launch_bounds__(256) void k_ftle(const double2* __restrict__ F, int nx, int ny,
-                                              double scaling, double dx, double dy,
-                                              const uint8_t* __restrict__ mask,
-                                              double* __restrict__ ftle) {
-    const int j = blockIdx.x * 64 + threadIdx.x;
-    const int i = blockIdx.y * 4 + threadIdx.y;
-    if (i >= nx || j >= ny) return;
-    const long k = static_cast<long>(i) * ny + j;
-    double out = 0.0;
-    if (interior(i, j, nx, ny) && !(mask != nullptr && mask[k])) {
-        const Grad g = grad_at(F, k, ny, dx, dy);
-        const double off = g.dxdx * g.dxdy + g.dydx * g.dydy;
-        const double a = g.dxdx * g.dxdx + g.dydx * g.dydx;
-        const double d = g.dxdy * g.dxdy + g.dydy * g.dydy;
-        const double trace = a + d;
-        const double disc = sqrt((a - d) * (a - d) + 4 * (off * off));
-        const double lmax = 0.5 * (trace + disc);
-        if (lmax > 1) out = scaling * log(lmax);
+// Row strips: one wavefront walks down R rows of a 62-column strip (lane l
+// holds column j0 - 1 + l; lanes 0 and 63 are the strip's halo), keeping
+// the rows above and below its point in registers and taking the column
+// neighbours from the adjacent lanes, so each flow-map element is read from
+// memory once per strip (plus one halo row at each end) -- the 16 B per point
+// of the algorithmic count -- where 64 x 4 blocks re-read the rows shared
+// with the blocks above and below.  Gradient as gradF_stencil_2D
+// (utils.py:41-44, the e/w/n/s of grad_at): results equal the point form's.
+template <bool EIG>
+__global__ __launch_bounds__(64) void k_ftle_strip(const double2* __restrict__ F, int nx, int ny,
+                                                   int R, double tscale, double dx, double dy,
+                                                   const uint8_t* __restrict__ mask,
+                                                   double* __restrict__ ftle,
+                                                   double2* __restrict__ evec) {
+    const int lane = static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.x) * 62 - 1 + lane;
+    const int i0 = static_cast<int>(blockIdx.y) * R;
+    const int i1 = i0 + R < nx ? i0 + R : nx;
+    const bool jin = j >= 0 && j < ny;
+    const bool writer = lane >= 1 && lane <= 62 && jin;
+    auto row = [&](int i) {
+        return (jin && i >= 0 && i < nx) ? F[static_cast<long>(i) * ny + j]
+                                         : make_double2(0.0, 0.0);
+    };
+    double2 up = row(i0 - 1), cur = row(i0);
+    for (int i = i0; i < i1; ++i) {
+        const double2 down = row(i + 1);
+        double2 n, s;
+        n.x = __shfl_down(cur.x, 1);
+        n.y = __shfl_down(cur.y, 1);
+        s.x = __shfl_up(cur.x, 1);
+        s.y = __shfl_up(cur.y, 1);
+        if (writer) {
+            const long k = static_cast<long>(i) * ny + j;
+            const bool in = interior(i, j, nx, ny) && !(mask != nullptr && mask[k]);
+            Grad g;
+            g.dxdx = (down.x - up.x) / (2 * dx);
+            g.dxdy = (n.x - s.x) / (2 * dy);
+            g.dydx = (down.y - up.y) / (2 * dx);
+            g.dydy = (n.y - s.y) / (2 * dy);
+            if constexpr (EIG) {
+                double f = 0.0;
+                double2 e = make_double2(0.0, 0.0);
+                if (in) ftle_eig_of(g, tscale, f, e);
+                ftle[k] = f;
+                evec[k] = e;
+            } else {
+                ftle[k] = in ? ftle_of(g, tscale) : 0.0;
+            }
+        }
+        up = cur;
+        cur = down;
     }
-    ftle[k] = out;
+}
+
+// Rows per strip: enough strips to fill the machine (about 4096
+// wavefronts), at most 64 rows (one halo row re-read per 64).
+inline int strip_rows(int nx, int ny) {
+    const long cols = (ny + 61) / 62;
+    long r = (static_cast<long>(nx) * cols + 4095) / 4096;
+    r = r < 4 ? 4 : (r > 64 ? 64 : r);
+    return static_cast<int>(r);
 }
 
 __global__ __launch_bounds__(256) void k_C_eig(const double2* __restrict__ F, int nx, int ny,
@@ -60,29 +102,6 @@ __global__ __launch_bounds__(256) void k_C_eig(const double2* __restrict__ F, in
     }
     evals[k] = make_double2(ev[0], ev[1]);
     evecs[k] = make_double4(V[0][0], V[0][1], V[1][0], V[1][1]);
-}
-
-// C_eig_2D + ftle_from_eig fused: the ridge path's FTLE comes from the
-// eigen-solver's largest eigenvalue (diagnostics.py:266-269).
-__global__ __launch_bounds__(256) void k_ftle_eig(const double2* __restrict__ F, int nx, int ny,
-                                                  double two_abs_T, double dx, double dy,
-                                                  const uint8_t* __restrict__ mask,
-                                                  double* __restrict__ ftle,
-                                                  double2* __restrict__ evec_max) {
-    const int j = blockIdx.x * 64 + threadIdx.x;
-    const int i = blockIdx.y * 4 + threadIdx.y;
-    if (i >= nx || j >= ny) return;
-    const long k = static_cast<long>(i) * ny + j;
-    double ev[2] = {0.0, 0.0}, V[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-    if (interior(i, j, nx, ny) && !(mask != nullptr && mask[k])) {
-        const Grad g = grad_at(F, k, ny, dx, dy);
-        const double off = g.dxdx * g.dxdy + g.dydx * g.dydy;
-        const double a = g.dxdx * g.dxdx + g.dydx * g.dydx;
-        const double d = g.dxdy * g.dxdy + g.dydy * g.dydy;
-        eig2_sym(a, off, d, ev, V);
-    }
-    ftle[k] = ev[1] > 1 ? log(ev[1]) / two_abs_T : 0.0;
-    evec_max[k] = make_double2(V[0][1], V[1][1]);
 }
 
 // Auxiliary-grid stencils: F is (nx, ny, naux) double2, the particles of
@@ -170,9 +189,11 @@ int nbx_ftle_grid_2d(const double* fm, int nx, int ny, double T, double dx, doub
     NBX_REQUIRE(nx >= 0 && ny >= 0, "negative grid size");
     if (static_cast<int64_t>(nx) * ny == 0) return 0;
     NBX_REQUIRE(fm != nullptr && ftle != nullptr, "NULL array");
-    const dim3 block(64, 4), grid((ny + 63) / 64, (nx + 3) / 4);
-    k_ftle<<<grid, block, 0, as_stream(stream)>>>(reinterpret_cast<const double2*>(fm), nx, ny,
-                                                  1 / (2 * fabs(T)), dx, dy, mask, ftle);
+    const int R = strip_rows(nx, ny);
+    const dim3 grid((ny + 61) / 62, (nx + R - 1) / R);
+    k_ftle_strip<false><<<grid, 64, 0, as_stream(stream)>>>(
+        reinterpret_cast<const double2*>(fm), nx, ny, R, 1 / (2 * fabs(T)), dx, dy, mask, ftle,
+        nullptr);
     NBX_LAUNCHED("ftle");
     return 0;
 }
@@ -196,9 +217,10 @@ int nbx_ftle_eig_2d(const double* fm, int nx, int ny, double T, double dx, doubl
     NBX_REQUIRE(nx >= 0 && ny >= 0, "negative grid size");
     if (static_cast<int64_t>(nx) * ny == 0) return 0;
     NBX_REQUIRE(fm != nullptr && ftle != nullptr && eigvec_max != nullptr, "NULL array");
-    const dim3 block(64, 4), grid((ny + 63) / 64, (nx + 3) / 4);
-    k_ftle_eig<<<grid, block, 0, as_stream(stream)>>>(
-        reinterpret_cast<const double2*>(fm), nx, ny, 2 * fabs(T), dx, dy, mask, ftle,
+    const int R = strip_rows(nx, ny);
+    const dim3 grid((ny + 61) / 62, (nx + R - 1) / R);
+    k_ftle_strip<true><<<grid, 64, 0, as_stream(stream)>>>(
+        reinterpret_cast<const double2*>(fm), nx, ny, R, 2 * fabs(T), dx, dy, mask, ftle,
         reinterpret_cast<double2*>(eigvec_max));
     NBX_LAUNCHED("ftle_eig");
     return 0;

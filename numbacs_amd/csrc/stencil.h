@@ -69,11 +69,9 @@ __device__ __forceinline__ bool interior(int i, int j, int nx, int ny) {
     return i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1;
 }
 
-// FTLE at one interior point (diagnostics.py:56-63): C = gradF^T gradF,
-// lmax by eigvalsh_max_2D's closed form (utils.py:185-189).
-__device__ __forceinline__ double ftle_point(const double2* __restrict__ F, long k, int ny,
-                                             double scaling, double dx, double dy) {
-    const Grad g = grad_at(F, k, ny, dx, dy);
+// FTLE of one interior point from its flow-map gradient (diagnostics.py:
+// 56-63: C = gradF^T gradF, its largest eigenvalue, log / (2|T|) if > 1).
+__device__ __forceinline__ double ftle_of(const Grad& g, double scaling) {
     const double off = g.dxdx * g.dxdy + g.dydx * g.dydy;
     const double a = g.dxdx * g.dxdx + g.dydx * g.dydx;
     const double d = g.dxdy * g.dxdy + g.dydy * g.dydy;
@@ -81,6 +79,25 @@ __device__ __forceinline__ double ftle_point(const double2* __restrict__ F, long
     const double disc = sqrt((a - d) * (a - d) + 4 * (off * off));
     const double lmax = 0.5 * (trace + disc);
     return lmax > 1 ? scaling * log(lmax) : 0.0;
+}
+
+// The same point's eigen-decomposition (C_eig_2D) and ftle_from_eig.
+__device__ __forceinline__ void ftle_eig_of(const Grad& g, double two_abs_T, double& ftle,
+                                            double2& evec) {
+    double ev[2], V[2][2];
+    const double off = g.dxdx * g.dxdy + g.dydx * g.dydy;
+    const double a = g.dxdx * g.dxdx + g.dydx * g.dydx;
+    const double d = g.dxdy * g.dxdy + g.dydy * g.dydy;
+    eig2_sym(a, off, d, ev, V);
+    ftle = ev[1] > 1 ? log(ev[1]) / two_abs_T : 0.0;
+    evec = make_double2(V[0][1], V[1][1]);
+}
+
+// FTLE at one interior point (diagnostics.py:56-63): C = gradF^T gradF,
+// lmax by eigvalsh_max_2D's closed form (utils.py:185-189).
+__device__ __forceinline__ double ftle_point(const double2* __restrict__ F, long k, int ny,
+                                             double scaling, double dx, double dy) {
+    return ftle_of(grad_at(F, k, ny, dx, dy), scaling);
 }
 
 // C_eig_2D at one interior point: ascending eigenvalues and eigenvectors.

@@ -172,3 +172,50 @@ def test_fp32_displacement_error_double_gyre(lib):
     err = np.abs(ft - ftr) / np.abs(ftr).max()
     assert np.sqrt(np.mean(err ** 2)) < 1e-4
     assert np.mean(err <= 1e-3) > 0.999
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("case", ["gridded", "wrap_nearest_mask"])
+def test_staged_launch_equals_grid_launch(lib, restore_config, precision, case):
+    """The LDS-staged launch (coefficient boxes copied to LDS per step,
+    persistent -1) runs the same arithmetic as the grid launch: equal flow
+    maps, status and step counts, on the gridded workload and on a global
+    field whose boxes straddle the longitude wrap, with nearest
+    extrapolation and a mask."""
+    from numbacs_amd import autotune
+    from numbacs_amd.flows import get_flow_2D, get_interp_arrays_2D
+    from numbacs_amd.integration import flowmap_grid_2d_device
+    from numbacs_amd.workloads import particle_axes, synthetic_velocity
+
+    if case == "gridded":
+        t, lon, lat, U, V = synthetic_velocity()
+        grid, Cu, Cv = get_interp_arrays_2D(t, lon, lat, U, V)
+        f = get_flow_2D(grid, Cu, Cv, spherical=1)
+        x, y = particle_axes()
+        mask, t0, T = None, 72.0, -72.0
+    else:
+        t = np.linspace(0.0, 48.0, 25)
+        lon = np.linspace(-180.0, 179.0, 360)
+        lat = np.linspace(-60.0, 60.0, 121)
+        TT, LON, LAT = np.meshgrid(t, lon, lat, indexing="ij")
+        U = 60.0 + 20.0 * np.sin(np.deg2rad(3 * LON) + 0.1 * TT) * np.cos(np.deg2rad(2 * LAT))
+        V = 15.0 * np.cos(np.deg2rad(2 * LON) - 0.05 * TT)
+        grid, Cu, Cv = get_interp_arrays_2D(t, lon, lat, U, V)
+        f = get_flow_2D(grid, Cu, Cv, spherical=1, extrap_mode="nearest")
+        x, y = np.linspace(150.0, 200.0, 121), np.linspace(-50.0, 50.0, 83)
+        mask = np.zeros((len(x), len(y)), bool)
+        mask[::7, ::5] = True
+        t0, T = 40.0, -36.0
+    xd, yd = lib.to_dev(x), lib.to_dev(y)
+    md = None if mask is None else lib.to_dev(mask.astype(np.uint8), np.uint8)
+    res = []
+    for cfg in ((4, 64, 0), (4, 64, -1)):
+        autotune.set_config(lib.NBX_FLOW_SPLINE3, precision, cfg)
+        st = lib.empty((len(x), len(y)), "i32")
+        ns = lib.empty((len(x), len(y)), "i32")
+        fm = flowmap_grid_2d_device(f, t0, T, xd, yd, [-1.0], maskd=md, precision=precision,
+                                    status=st, nsteps=ns)
+        res.append((lib.to_host(fm), lib.to_host(st), lib.to_host(ns)))
+    assert (res[0][1] != 0).any()
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)

@@ -236,13 +236,17 @@ def test_precision_study_fp32_vs_fp64(lib, case):
         p = bickley_params(-1.0)
         x, y = bickley_grid(2001, 801)
         # chaotic over 40 days: rounding is amplified as any perturbation
-        bars = dict(rms=3e-2, within=0.6, ridge=0.35)
+        # (measured: RMS 8.6e-3, 82% within 1e-3, ridge points differing 6.6%)
+        bars = dict(rms=2e-2, within=0.75, ridge=0.1)
     else:
         name, T = "double_gyre", (-10.0 if case == "dg10" else -20.0)
         p = double_gyre_params(-1.0)
         x, y = np.linspace(0, 2, 2001), np.linspace(0, 1, 1001)
-        bars = dict(rms=1e-3 if case == "dg10" else 1e-2, within=0.995 if case == "dg10" else
-                    0.99, ridge=0.02 if case == "dg10" else 0.05)
+        # measured: T=-10 RMS 1.0e-4, 99.95% within 1e-3, ridge points
+        # differing 4.5% (many sit on the sub-pixel test's edge at this
+        # horizon); T=-20 RMS 2.3e-3, 99.76%, 0.55%
+        bars = (dict(rms=2e-4, within=0.999, ridge=0.06) if case == "dg10" else
+                dict(rms=5e-3, within=0.995, ridge=0.01))
     f = get_predefined_flow(name, return_default_params=False, return_domain=False)
     xd, yd = lib.to_dev(x), lib.to_dev(y)
     dx, dy = x[1] - x[0], y[1] - y[0]
