@@ -43,9 +43,9 @@ __global__ __launch_bounds__(64) void k_ftle_strip(const double2* __restrict__ F
         return (jin && i >= 0 && i < nx) ? F[static_cast<long>(i) * ny + j]
                                          : make_double2(0.0, 0.0);
     };
-    double2 up = row(i0 - 1), cur = row(i0);
-    for (int i = i0; i < i1; ++i) {
-        const double2 down = row(i + 1);
+    // the point of row i from its rows above and below and its own row
+    // (column neighbours from the adjacent lanes; every lane shuffles)
+    auto emit = [&](int i, const double2& up, const double2& cur, const double2& down) {
         double2 n, s;
         n.x = __shfl_down(cur.x, 1);
         n.y = __shfl_down(cur.y, 1);
@@ -69,6 +69,22 @@ __global__ __launch_bounds__(64) void k_ftle_strip(const double2* __restrict__ F
                 ftle[k] = in ? ftle_of(g, tscale) : 0.0;
             }
         }
+    };
+    double2 up = row(i0 - 1), cur = row(i0);
+    int i = i0;
+    // four rows a pass: their loads are in flight together
+    for (; i + 4 <= i1; i += 4) {
+        const double2 d0 = row(i + 1), d1 = row(i + 2), d2 = row(i + 3), d3 = row(i + 4);
+        emit(i, up, cur, d0);
+        emit(i + 1, cur, d0, d1);
+        emit(i + 2, d0, d1, d2);
+        emit(i + 3, d1, d2, d3);
+        up = d2;
+        cur = d3;
+    }
+    for (; i < i1; ++i) {
+        const double2 down = row(i + 1);
+        emit(i, up, cur, down);
         up = cur;
         cur = down;
     }
