@@ -302,6 +302,14 @@ struct Spline3 {
     int spherical, extrap;
     double r;     // sphere radius (km) for spherical = 1, 2
     double p0;    // integration direction
+    // 4-lane unstaged form: the lane's column of the last stencil it
+    // gathered and that stencil's origin (element offset), so a query in
+    // the same cell as the particle's previous one reads registers instead
+    // of the cache hierarchy (the kernel's own copy of the functor holds it)
+    static constexpr bool kReuse = (G == 4) && !S;
+    static constexpr int kRows = sizeof(R) == sizeof(double) ? 16 : 8;
+    mutable P cst[kReuse ? kRows : 1];
+    mutable long ckey = -1;
 
     __device__ __forceinline__ void cell(int d, double q, int& i, R& s) const {
         const double u = (q - lo[d]) * dinv[d];
@@ -410,6 +418,40 @@ struct Spline3 {
                     const P* lb = box.lds + ((it - box.it0) * box.bx + (ix - box.ix0)) * box.by +
                                   (iy - box.iy0) + g;
                     column_sums(lb, box.bx * box.by, box.by, wt, wx, au, av);
+                }
+            } else if constexpr (kReuse) {
+                const long key = static_cast<long>(it) * st + static_cast<long>(ix) * sx + iy;
+                if (key != ckey) {
+                    ckey = key;
+                    const P* lb = base + g;
+#pragma unroll
+                    for (int a = 0; a < kRows / 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b)
+                            cst[a * 4 + b] = lb[(sizeof(R) == sizeof(double) ? a : 2 * a) * st +
+                                                b * sx];
+                }
+                // the same sums as column_sums over the held rows
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    R ru = 0, rv = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        R cu, cv;
+                        if constexpr (sizeof(R) == sizeof(double)) {
+                            const P e = cst[a * 4 + b];
+                            cu = e.x;
+                            cv = e.y;
+                        } else {
+                            const P e = cst[(a >> 1) * 4 + b];
+                            cu = (a & 1) ? e.z : e.x;
+                            cv = (a & 1) ? e.w : e.y;
+                        }
+                        ru = fma(wx[b], cu, ru);
+                        rv = fma(wx[b], cv, rv);
+                    }
+                    au = fma(wt[a], ru, au);
+                    av = fma(wt[a], rv, av);
                 }
             } else {
                 column_sums(base + g, st, sx, wt, wx, au, av);
