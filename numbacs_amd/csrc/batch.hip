@@ -1,8 +1,8 @@
 // Batched FTLE time series: many flow-map requests (each its own window
 // t0, T and particle grid) over one flow, integrated by one launch -- the
-// persistent work queue, or one non-persistent launch over every request's
-// particle tiles (the launch configuration's persistent occupancy 0; the
-// autotuner picks the faster) -- followed by one batched FTLE launch over
+// persistent queue of particle tiles, or one non-persistent launch over
+// every request's tiles (the launch configuration's persistent occupancy 0;
+// the autotuner picks the faster) -- followed by one batched FTLE launch over
 // every request's grid.  The pair can be captured once into a HIP graph and
 // replayed.  Requests run longest window first (the tile launch dispatches
 // blocks in order, so the longest solves do not form the launch's tail).
@@ -71,10 +71,10 @@ struct nbx_batch {
     int64_t total = 0;
     int ntiles = 0;
     nbx::ReqDev* d_table = nullptr;
-    int64_t* d_offsets = nullptr;
     int64_t total_slots = 0;
-    int64_t* d_slot_off = nullptr;  // tile launch: first slot of each request
-    unsigned long long* d_counter = nullptr;
+    int64_t* d_slot_off = nullptr;  // first slot of each request (tile order)
+    char* d_chunk = nullptr;        // queue launch: ring, counters, solver states
+    size_t chunk_bytes = 0;
     nbx::FtleReqDev* d_ftle = nullptr;
     nbx::TileDev* d_tiles = nullptr;
     hipGraph_t graph = nullptr;
@@ -96,13 +96,48 @@ int launch_batch_queue(nbx_batch* b, const Flow& f, const LaunchCfg& cfg, hipStr
     }
     const int cus = device_cu_count();
     NBX_REQUIRE(cus > 0, "cannot query the compute-unit count");
-    const int per_cu = cfg.persistent;
-    const int64_t want = (b->total * G + block - 1) / block;
-    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(want, static_cast<int64_t>(cus) * per_cu));
-    k_flowmap_queue<Flow, R, G><<<static_cast<unsigned>(grid), block, 0, s>>>(
-        f, ReqDev{}, b->d_table, b->d_offsets, b->nreq, b->total, b->d_counter, b->rtol, b->atol,
-        b->nmax);
-    NBX_LAUNCHED("flowmap_queue (batch)");
+    if (b->total_slots == 0) return 0;
+    const int64_t ntiles = b->total_slots / (64 / G);
+    const unsigned long long ring = chunk_ring(ntiles);
+    const size_t state_off = ring * sizeof(unsigned long long) + 256;
+    NBX_REQUIRE(b->d_chunk != nullptr &&
+                    b->chunk_bytes >= state_off + sizeof(Dop853State<R>) * b->total_slots,
+                "the batch's queue buffers are not allocated for this launch configuration");
+    char* w = b->d_chunk;
+    const ChunkQueue q{reinterpret_cast<unsigned long long*>(w),
+                       reinterpret_cast<unsigned long long*>(w + state_off - 256),
+                       reinterpret_cast<unsigned long long*>(w + state_off - 128),
+                       reinterpret_cast<unsigned long long*>(w + state_off - 64), ring - 1,
+                       static_cast<unsigned long long>(ntiles)};
+    k_chunk_init<<<blocks_for(static_cast<int64_t>(ring), 256), 256, 0, s>>>(q, ntiles);
+    NBX_LAUNCHED("chunk_init (batch)");
+    const int64_t want = (ntiles * 64 + block - 1) / block;
+    const int64_t grid =
+        std::max<int64_t>(1, std::min<int64_t>(want, static_cast<int64_t>(cus) * cfg.persistent));
+    k_flowmap_chunks<Flow, R, G><<<static_cast<unsigned>(grid), block, 0, s>>>(
+        f, ReqDev{}, b->d_table, b->d_slot_off, b->nreq, q,
+        reinterpret_cast<Dop853State<R>*>(w + state_off), b->rtol, b->atol, b->nmax);
+    NBX_LAUNCHED("flowmap_chunks (batch)");
+    return 0;
+}
+
+// Allocate (or grow) the queue launch's buffers for the current launch
+// configuration: before a run, and before a capture (no allocation may
+// happen inside one).  Sized for the FP64 state, the larger of the two.
+int ensure_chunk_buffers(nbx_batch* b) {
+    const LaunchCfg cfg = launch_cfg(b->flow->kind, b->precision);
+    if (cfg.persistent <= 0 || b->total_slots == 0) return 0;
+    const int G = (b->flow->kind == NBX_FLOW_SPLINE3 && b->flow->extrap != 1) ? cfg.lanes : 1;
+    const int64_t ntiles = b->total_slots / (64 / G);
+    NBX_REQUIRE(ntiles < (int64_t(1) << 31), "too many particle tiles for the queue");
+    const size_t need = chunk_ring(ntiles) * sizeof(unsigned long long) + 256 +
+                        sizeof(Dop853State<double>) * static_cast<size_t>(b->total_slots);
+    if (b->chunk_bytes >= need) return 0;
+    if (b->d_chunk) (void)hipFree(b->d_chunk);
+    b->d_chunk = nullptr;
+    b->chunk_bytes = 0;
+    NBX_HIP(hipMalloc(&b->d_chunk, need));
+    b->chunk_bytes = need;
     return 0;
 }
 
@@ -132,8 +167,6 @@ int launch_batch_flow(nbx_batch* b, hipStream_t s) {
 }
 
 int enqueue_batch(nbx_batch* b, hipStream_t s) {
-    k_queue_reset<<<1, 1, 0, s>>>(b->d_counter);
-    NBX_LAUNCHED("queue_reset");
     int rc = b->precision == NBX_FP32_DISP ? launch_batch_flow<float>(b, s)
                                            : launch_batch_flow<double>(b, s);
     if (rc) return rc;
@@ -148,9 +181,8 @@ void free_batch(nbx_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_table) (void)hipFree(b->d_table);
-    if (b->d_offsets) (void)hipFree(b->d_offsets);
     if (b->d_slot_off) (void)hipFree(b->d_slot_off);
-    if (b->d_counter) (void)hipFree(b->d_counter);
+    if (b->d_chunk) (void)hipFree(b->d_chunk);
     if (b->d_ftle) (void)hipFree(b->d_ftle);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
     delete b;
@@ -218,17 +250,13 @@ int nbx_batch_create(const nbx_flow* flow, const nbx_request* reqs, int nreq,
     b->total_slots = slot_off[nreq];
     b->ntiles = static_cast<int>(tiles.size());
     bool ok = hipMalloc(&b->d_table, sizeof(ReqDev) * nreq) == hipSuccess &&
-              hipMalloc(&b->d_offsets, sizeof(int64_t) * (nreq + 1)) == hipSuccess &&
-              hipMalloc(&b->d_slot_off, sizeof(int64_t) * (nreq + 1)) == hipSuccess &&
-              hipMalloc(&b->d_counter, sizeof(unsigned long long)) == hipSuccess;
+              hipMalloc(&b->d_slot_off, sizeof(int64_t) * (nreq + 1)) == hipSuccess;
     if (ok && !ftle.empty())
         ok = hipMalloc(&b->d_ftle, sizeof(FtleReqDev) * ftle.size()) == hipSuccess &&
              hipMalloc(&b->d_tiles, sizeof(TileDev) * tiles.size()) == hipSuccess;
     if (ok)
         ok = hipMemcpy(b->d_table, table.data(), sizeof(ReqDev) * nreq, hipMemcpyHostToDevice) ==
                  hipSuccess &&
-             hipMemcpy(b->d_offsets, offsets.data(), sizeof(int64_t) * (nreq + 1),
-                       hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(b->d_slot_off, slot_off.data(), sizeof(int64_t) * (nreq + 1),
                        hipMemcpyHostToDevice) == hipSuccess;
     if (ok && !ftle.empty())
@@ -249,6 +277,7 @@ int64_t nbx_batch_particles(const nbx_batch* b) { return b ? b->total : -1; }
 
 int nbx_batch_run(nbx_batch* b, nbx_stream stream) {
     NBX_REQUIRE(b != nullptr, "batch is NULL");
+    if (const int rc = ensure_chunk_buffers(b)) return rc;
     return enqueue_batch(b, as_stream(stream));
 }
 
@@ -264,6 +293,7 @@ int nbx_batch_capture(nbx_batch* b, nbx_stream stream) {
         (void)hipGraphDestroy(b->graph);
         b->graph = nullptr;
     }
+    if (const int rc = ensure_chunk_buffers(b)) return rc;
     NBX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_batch(b, s);
     hipGraph_t g = nullptr;

@@ -533,20 +533,31 @@ int launch_solve(const Flow& f, const Src& src, const SolveArgs& a, const Launch
                  hipStream_t s) {
     const int block = cfg.block;
     if constexpr (std::is_same<Src, GridSource>::value) {
-        if (cfg.persistent > 0) {
+        if (cfg.persistent > 0) {  // persistent grid over the queue of particle tiles
             const int cus = device_cu_count();
             NBX_REQUIRE(cus > 0, "cannot query the compute-unit count");
-            const int64_t want = (a.P * G + block - 1) / block;
+            const int64_t slots = src.slots();
+            if (slots == 0) return 0;
+            const int64_t ntiles = slots / (64 / G);
+            NBX_REQUIRE(ntiles < (int64_t(1) << 31), "too many particle tiles for the queue");
+            const unsigned long long ring = chunk_ring(ntiles);
+            const size_t state_off = ring * sizeof(unsigned long long) + 256;
+            Workspace ws(s, 3, state_off + sizeof(Dop853State<R>) * static_cast<size_t>(slots));
+            if (ws.error()) return ws.error();
+            char* w = ws.get();
+            const ChunkQueue q{reinterpret_cast<unsigned long long*>(w),
+                               reinterpret_cast<unsigned long long*>(w + state_off - 256),
+                               reinterpret_cast<unsigned long long*>(w + state_off - 128),
+                               reinterpret_cast<unsigned long long*>(w + state_off - 64),
+                               ring - 1, static_cast<unsigned long long>(ntiles)};
+            k_chunk_init<<<blocks_for(static_cast<int64_t>(ring), 256), 256, 0, s>>>(q, ntiles);
+            NBX_LAUNCHED("chunk_init");
+            const int64_t want = (ntiles * 64 + block - 1) / block;
             const int64_t grid = std::min<int64_t>(want, static_cast<int64_t>(cus) * cfg.persistent);
-            unsigned long long* counter = nullptr;
-            NBX_HIP(hipMallocAsync(reinterpret_cast<void**>(&counter), sizeof(*counter), s));
-            k_queue_reset<<<1, 1, 0, s>>>(counter);
-            NBX_LAUNCHED("queue_reset");
-            k_flowmap_queue<Flow, R, G><<<static_cast<unsigned>(grid), block, 0, s>>>(
-                f, single_request(src, a), nullptr, nullptr, 1, a.P, counter, a.rtol, a.atol,
-                a.nmax);
-            NBX_LAUNCHED("flowmap_queue");
-            NBX_HIP(hipFreeAsync(counter, s));
+            k_flowmap_chunks<Flow, R, G><<<static_cast<unsigned>(grid), block, 0, s>>>(
+                f, single_request(src, a), nullptr, nullptr, 1, q,
+                reinterpret_cast<Dop853State<R>*>(w + state_off), a.rtol, a.atol, a.nmax);
+            NBX_LAUNCHED("flowmap_chunks");
             return 0;
         }
     }

@@ -1,18 +1,35 @@
-// Persistent DOP853 launch with a device work queue.
+// Persistent DOP853 launch over a device queue of particle tiles.
 //
-// A grid sized to the machine (not to the problem) pulls particles from a
-// global counter.  Every lane group (G lanes per particle) that has finished
-// its solve -- or has none yet -- takes the next particle in the same pass of
-// the loop, so the adaptive-step imbalance between particles (step counts
-// differ by 2-6x on the reference workloads) no longer leaves lanes idle
-// until the slowest particle of their wavefront finishes.  Fetches are
-// aggregated per wavefront: one atomicAdd claims a run of indices for all
-// idle groups of the wave.
+// A wavefront's particles finish together only when their step counts
+// agree, and a grid launch dispatches wavefronts in order: the slowest tiles
+// that start late form the launch's tail (on the gridded workload the
+// in-order schedule is ~1.28x the balanced one, and launching tiles longest
+// first measured 11% faster in FP64, 19% in FP32).  Step counts are not
+// known before the solve, so the queue balances instead: a wavefront pops a
+// tile (one wavefront's particles), runs at most kChunkSteps attempted steps
+// per particle, and when some particle is still running stores the tile's
+// solver states and pushes the tile to the back of the queue.  Long tiles
+// are thus spread over whichever wavefronts are free, in chunks.
+//
+// The queue is a ring of 64-bit entries (reservation index << 32 | payload)
+// with a claim counter (head), a push counter (tail) and a count of finished
+// tiles (done).  A wavefront claims ring index h by fetch-and-add on head
+// (one atomic per claim, whatever the number of wavefronts: a
+// compare-and-swap claim measured 50-150x slower with two thousand
+// wavefronts retrying) and takes the entry of slot h once it carries
+// reservation index h; until then it sleeps and polls, and it leaves when
+// every tile has finished.  A claimed index is always filled unless all
+// tiles finish first: every unfinished tile is either in the ring or held
+// by a running wavefront that will push it, and pushes take the reservation
+// indices in order.  The ring holds at least four times as many entries as
+// there are tiles (each tile is in the ring at most once), so a slot's next
+// reservation comes long after its entry has been taken.  Each solve is
+// bounded by max_steps, so the tiles finish and every wavefront reaches the
+// exit; a poll limit of about ten seconds bounds the wait regardless.
 //
 // The queue spans a table of requests (flow-map grids with their own window
 // t0, T and particle axes), which is how a whole FTLE time series runs in one
-// launch.  Each solve is bounded by max_steps and the queue by its total, so
-// every wavefront reaches the exit.
+// launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -63,82 +80,175 @@ struct ReqDev {
     int32_t* nsteps;              // nullable
 };
 
-static __global__ void k_queue_reset(unsigned long long* counter) { *counter = 0ull; }
+// Attempted steps per particle before a wavefront hands its tile back.
+constexpr int kChunkSteps = 16;
+
+struct ChunkQueue {
+    unsigned long long* items;  // ring: (reservation index << 32) | payload
+    unsigned long long* head;   // next reservation index to pop
+    unsigned long long* tail;   // next reservation index to push
+    unsigned long long* done;   // tiles whose particles have all finished
+    unsigned long long mask;    // ring entries - 1 (a power of two above the tile count)
+    unsigned long long ntiles;
+};
+// Polls of an unfilled claim before a wavefront gives up waiting (about ten
+// seconds at ~1 us of sleep per poll): a safety bound, far beyond any solve
+// here; a wavefront that gives up drops its claim.
+constexpr int kMaxPolls = 10 << 20;
+// payload bit 31: the tile has been started (its states are stored)
+constexpr unsigned kResume = 0x80000000u;
+
+// Ring entries for ntiles tiles: a power of two >= 4 ntiles (at least 1024).
+inline unsigned long long chunk_ring(int64_t ntiles) {
+    unsigned long long n = 1024;
+    while (n < 4 * static_cast<unsigned long long>(ntiles)) n <<= 1;
+    return n;
+}
+
+// Entries 0..ntiles-1 hold tiles 0..ntiles-1 (reservations 0..ntiles-1);
+// the others carry an index no reservation of their slot can have (the
+// complement of the slot number: odd minus even never matches).  A kernel
+// rather than a memset, so the reset can be captured in a graph.
+static __global__ void k_chunk_init(ChunkQueue q, int64_t ntiles) {
+    const unsigned long long i = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x;
+    if (i > q.mask) return;
+    q.items[i] = i < static_cast<unsigned long long>(ntiles)
+                     ? ((i & 0xffffffffull) << 32) | i
+                     : ((~i & 0xffffffffull) << 32);
+    if (i == 0) {
+        *q.head = 0ull;
+        *q.tail = static_cast<unsigned long long>(ntiles);
+        *q.done = 0ull;
+    }
+}
+
+// Memory ordering without cache-wide fences.  An agent-scope acquire or
+// release writes back or invalidates the whole L2 of an XCD, which drops
+// the spline coefficients every wavefront there is gathering (measured:
+// 40-270x slower).  Instead the handed-on solver states are stored and
+// loaded word by word as agent-scope relaxed atomics, which reach the
+// point of coherence of all XCDs per access; a workgroup-scope release
+// fence (a wait for the wavefront's stores to complete, and a compiler
+// barrier) orders them before the ring entry that publishes the tile, and
+// a reader's state addresses depend on the entry it popped.
+template <class T>
+__device__ __forceinline__ void coherent_store(T* dst, const T& v) {
+    static_assert(sizeof(T) % 8 == 0 && alignof(T) >= 8, "8-byte words");
+    unsigned long long w[sizeof(T) / 8];
+    __builtin_memcpy(w, &v, sizeof(T));
+    auto* d = reinterpret_cast<unsigned long long*>(dst);
+#pragma unroll
+    for (int i = 0; i < static_cast<int>(sizeof(T) / 8); ++i)
+        __hip_atomic_store(d + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T coherent_load(const T* src) {
+    unsigned long long w[sizeof(T) / 8];
+    const auto* d = reinterpret_cast<const unsigned long long*>(src);
+#pragma unroll
+    for (int i = 0; i < static_cast<int>(sizeof(T) / 8); ++i)
+        w[i] = __hip_atomic_load(const_cast<unsigned long long*>(d + i), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
+// Claim the next ring index and wait for its tile: the payload, or -1 once
+// every tile has finished (or after kMaxPolls polls).
+__device__ __forceinline__ long long chunk_pop(const ChunkQueue& q) {
+    const unsigned long long h = atomicAdd(q.head, 1ull);
+    for (int polls = 0;; ++polls) {
+        const unsigned long long e =
+            __hip_atomic_load(q.items + (h & q.mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((e >> 32) == (h & 0xffffffffull)) return static_cast<long long>(e & 0xffffffffull);
+        if (__hip_atomic_load(q.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= q.ntiles ||
+            polls >= kMaxPolls)
+            return -1;
+        __builtin_amdgcn_s_sleep(32);
+    }
+}
+
+__device__ __forceinline__ void chunk_push(const ChunkQueue& q, unsigned payload) {
+    const unsigned long long r = atomicAdd(q.tail, 1ull);
+    __hip_atomic_store(q.items + (r & q.mask), ((r & 0xffffffffull) << 32) | payload,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
 
+// The queue's tiles are the grid launch's wavefronts: 64 / G consecutive
+// slots in tile order (a 4 x 4 particle tile at G = 4).  Slots are numbered
+// across the requests (request r owns [slot_off[r], slot_off[r+1]), a
+// multiple of 64 each, so no tile straddles two requests); state holds one
+// solver state per slot.  Masked particles are written at their tile's first
+// chunk; a particle's outputs are written when its solve ends.
 template <class Flow, class R, int G>
-__global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap_queue(
-    Flow f, ReqDev single, const ReqDev* __restrict__ table, const int64_t* __restrict__ offsets,
-    int nreq, int64_t total, unsigned long long* __restrict__ counter, double rtol, double atol,
+__global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : NBX_FLOWMAP_MIN_WAVES) void k_flowmap_chunks(
+    Flow f, ReqDev single, const ReqDev* __restrict__ table, const int64_t* __restrict__ slot_off,
+    int nreq, ChunkQueue q, Dop853State<R>* __restrict__ state, double rtol, double atol,
     long long nmax) {
+    constexpr int kSlots = 64 / G;
     const int lane = static_cast<int>(__lane_id());
-    const int glead = lane & ~(G - 1);
-    const bool is_lead = (lane & (G - 1)) == 0;
-    Dop853State<R> s;
-    s.idid = 0;
-    int64_t p = 0;
-    int r = 0;
-    bool active = false, exhausted = false;
+    const bool lead = (lane & (G - 1)) == 0;
     for (;;) {
-        // ---- refill (wavefront-uniform control) ----
-        const bool want = !active && !exhausted;
-        const unsigned long long m = __ballot(want && is_lead);
-        if (m != 0ull) {
-            const int first = __ffsll(static_cast<long long>(m)) - 1;
-            unsigned long long base = 0;
-            if (lane == first) base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(m)));
-            base = __shfl(base, first);
-            long long q = -1;
-            if (want && is_lead)
-                q = static_cast<long long>(base) +
-                    __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-            q = __shfl(q, glead);
-            if (want) {
-                if (q >= 0 && q < total) {
-                    if (table == nullptr) {
-                        r = 0;
-                        p = q;
-                    } else {
-                        int lo = 0, hi = nreq - 1;
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (offsets[mid] <= q) lo = mid;
-                            else hi = mid - 1;
-                        }
-                        r = lo;
-                        p = q - offsets[lo];
-                    }
-                    const ReqDev& Rq = table == nullptr ? single : table[r];
-                    if (Rq.mask != nullptr && Rq.mask[p]) {
-                        if (is_lead) {
-                            Rq.out[p] = make_double2(0.0, 0.0);
-                            if (Rq.status) Rq.status[p] = 0;
-                            if (Rq.nsteps) Rq.nsteps[p] = 0;
-                        }
-                    } else {
-                        const int64_t i = p / Rq.ny;
-                        dop853_init(f, s, Rq.s0, Rq.s1, Rq.xs[i], Rq.ys[p - i * Rq.ny], rtol, atol);
-                        active = true;
-                    }
-                } else {
-                    exhausted = true;
+        long long e = 0;
+        if (lane == 0) e = chunk_pop(q);
+        e = __shfl(e, 0);
+        if (e < 0) break;
+        const unsigned u = static_cast<unsigned>(e) & ~kResume;
+        const bool resume = (static_cast<unsigned>(e) & kResume) != 0;
+        const int64_t slot = static_cast<int64_t>(u) * kSlots + lane / G;
+        int r = 0;
+        if (table != nullptr) {
+            int lo = 0, hi = nreq - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (slot_off[mid] <= slot) lo = mid;
+                else hi = mid - 1;
+            }
+            r = lo;
+        }
+        const ReqDev& Rq = table == nullptr ? single : table[r];
+        const int64_t p =
+            tile_particle(slot - (table == nullptr ? 0 : slot_off[r]), Rq.nx, Rq.ny);
+        Dop853State<R> s;
+        bool active = false;
+        if (p >= 0) {
+            if (resume) {
+                s = coherent_load(state + slot);
+                active = s.idid == 0;
+            } else if (Rq.mask != nullptr && Rq.mask[p]) {
+                if (lead) {
+                    Rq.out[p] = make_double2(0.0, 0.0);
+                    if (Rq.status) Rq.status[p] = 0;
+                    if (Rq.nsteps) Rq.nsteps[p] = 0;
                 }
+                s.idid = 1;  // nothing left to do if the tile comes back
+            } else {
+                const int64_t i = p / Rq.ny;
+                dop853_init(f, s, Rq.s0, Rq.s1, Rq.xs[i], Rq.ys[p - i * Rq.ny], rtol, atol);
+                active = true;
             }
         }
-        if (__ballot(active || !exhausted) == 0ull) break;
-        // ---- one attempted step for every group with a particle ----
-        if (active) {
+        for (int n = 0; active && n < kChunkSteps; ++n) {
             if (!dop853_step(f, s, rtol, atol, nmax)) {
-                if (is_lead) {
-                    const ReqDev& Rq = table == nullptr ? single : table[r];
+                if (lead) {
                     Rq.out[p] = make_double2(s.px(), s.py());
                     if (Rq.status) Rq.status[p] = s.idid;
                     if (Rq.nsteps) Rq.nsteps[p] = s.nstep;
                 }
                 active = false;
             }
+        }
+        if (__ballot(active) != 0ull) {
+            if (lead && p >= 0) coherent_store(state + slot, s);
+            // the states complete before the entry that hands them on
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (lane == 0) chunk_push(q, u | kResume);
+        } else if (lane == 0) {
+            atomicAdd(q.done, 1ull);
         }
     }
 }

@@ -1,11 +1,13 @@
-"""Batched time series (persistent work-queue launch, batched FTLE, HIP
+"""Batched time series (persistent tile-queue launch, batched FTLE, HIP
 graph replay), the persistent single-grid launch and the FP32-displacement
 form, all through the C ABI.
 
 The queue kernel runs the same per-particle solver as the one-group-per-
-particle kernel, so with the same lanes-per-particle setting its flow maps
-must be bit-identical to flowmap_grid_2D's; FTLE fields from the batch must
-be bit-identical to ftle_grid_2D / the fused eigen stencil on those maps.
+particle kernel -- a tile handed back after a chunk of steps resumes from its
+stored solver states exactly -- so with the same lanes-per-particle setting
+its flow maps must be bit-identical to flowmap_grid_2D's; FTLE fields from
+the batch must be bit-identical to ftle_grid_2D / the fused eigen stencil on
+those maps.
 """
 import numpy as np
 import pytest
@@ -108,14 +110,18 @@ def test_batch_matches_single_grid_spline(lib, restore_config, small_field, lane
     b.close()
 
 
-def test_graph_replays_match_eager(lib, small_field):
+@pytest.mark.parametrize("cfg", [(4, 64, 0), (4, 64, 4)])
+def test_graph_replays_match_eager(lib, restore_config, small_field, cfg):
     """The captured graph replayed several times, with eager work and
-    synchronisation between replays, reproduces the eager batch exactly."""
+    synchronisation between replays, reproduces the eager batch exactly:
+    the tile launch, and the tile queue (whose ring is reset by a kernel in
+    the graph and whose tiles go round in chunks)."""
     import torch
 
     from numbacs_amd.batch import FTLEBatch
     from numbacs_amd.integration import flowmap_grid_2D
 
+    restore_config.set_config(2, 0, cfg)
     grid, Cu, Cv, f = small_field
     reqs = [dict(t0=20.0 + 2 * k, T=-12.0, x=np.linspace(-100, 35, 50), y=np.linspace(-5, 45, 31))
             for k in range(4)]
@@ -176,12 +182,13 @@ def test_fp32_displacement_error_double_gyre(lib):
 
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("case", ["gridded", "wrap_nearest_mask"])
-def test_staged_launch_equals_grid_launch(lib, restore_config, precision, case):
+def test_staged_and_queue_launches_equal_grid_launch(lib, restore_config, precision, case):
     """The LDS-staged launch (coefficient boxes copied to LDS per step,
-    persistent -1) runs the same arithmetic as the grid launch: equal flow
-    maps, status and step counts, on the gridded workload and on a global
-    field whose boxes straddle the longitude wrap, with nearest
-    extrapolation and a mask."""
+    persistent -1) and the persistent tile queue (tiles handed back every
+    16 steps with their solver states stored, persistent n > 0) run the same
+    arithmetic as the grid launch: equal flow maps, status and step counts,
+    on the gridded workload and on a global field whose boxes straddle the
+    longitude wrap, with nearest extrapolation and a mask."""
     from numbacs_amd import autotune
     from numbacs_amd.flows import get_flow_2D, get_interp_arrays_2D
     from numbacs_amd.integration import flowmap_grid_2d_device
@@ -209,7 +216,7 @@ def test_staged_launch_equals_grid_launch(lib, restore_config, precision, case):
     xd, yd = lib.to_dev(x), lib.to_dev(y)
     md = None if mask is None else lib.to_dev(mask.astype(np.uint8), np.uint8)
     res = []
-    for cfg in ((4, 64, 0), (4, 64, -1)):
+    for cfg in ((4, 64, 0), (4, 64, -1), (4, 64, 4), (4, 256, 2)):
         autotune.set_config(lib.NBX_FLOW_SPLINE3, precision, cfg)
         st = lib.empty((len(x), len(y)), "i32")
         ns = lib.empty((len(x), len(y)), "i32")
@@ -217,5 +224,6 @@ def test_staged_launch_equals_grid_launch(lib, restore_config, precision, case):
                                     status=st, nsteps=ns)
         res.append((lib.to_host(fm), lib.to_host(st), lib.to_host(ns)))
     assert (res[0][1] != 0).any()
-    for a, b in zip(res[0], res[1]):
-        assert np.array_equal(a, b)
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            assert np.array_equal(a, b)
