@@ -151,7 +151,11 @@ int launch_batch_flow(nbx_batch* b, hipStream_t s) {
             return launch_batch_queue<Spline3<R, 1, true>, R, 1>(b, make_spline<R, true>(fl, p[0]),
                                                                  cfg, s);
         const Spline3<R, 1> sp = make_spline<R, false>(fl, p[0]);
-        if (cfg.lanes == 4) return launch_batch_queue<Spline3<R, 4>, R, 4>(b, regroup<4>(sp), cfg, s);
+        if (cfg.lanes == 4) {
+            NBX_REQUIRE(sp.cbytes != 0, "the 4-lane spline kernel needs a coefficient array "
+                                        "under 4 GiB (set 1 or 2 lanes per particle)");
+            return launch_batch_queue<Spline3<R, 4>, R, 4>(b, regroup<4>(sp), cfg, s);
+        }
         if (cfg.lanes == 2) return launch_batch_queue<Spline3<R, 2>, R, 2>(b, regroup<2>(sp), cfg, s);
         return launch_batch_queue<Spline3<R, 1>, R, 1>(b, sp, cfg, s);
     }

@@ -45,6 +45,9 @@ Spline3<R, 1, L> make_spline(const nbx_flow* fl, double p0) {
     Spline3<R, 1, L> s;
     if constexpr (sizeof(R) == 8) s.C = fl->coef;
     else s.C = fl->coef32;
+    // 16-byte elements either way; 0 (never used) past 4 GiB
+    const int64_t bytes = fl->ncoef * 16;
+    s.cbytes = bytes < (int64_t(1) << 32) ? static_cast<unsigned>(bytes) : 0u;
     for (int d = 0; d < 3; ++d) {
         s.lo[d] = fl->grid9[3 * d];
         s.hi[d] = fl->grid9[3 * d + 1];
@@ -93,6 +96,7 @@ Spline3<R, G, false, S> regroup(const Spline3<R, 1, false>& a) {
     }
     s.sx = a.sx;
     s.st = a.st;
+    s.cbytes = a.cbytes;
     s.spherical = a.spherical;
     s.extrap = a.extrap;
     s.r = a.r;

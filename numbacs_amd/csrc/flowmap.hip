@@ -591,7 +591,11 @@ int dispatch_solve(const nbx_flow* fl, const double* params, int np, const Src& 
                 return 0;
             }
         }
-        if (cfg.lanes == 4) return launch_solve<Spline3<R, 4>, R, 4>(regroup<4>(sp), src, a, cfg, s);
+        if (cfg.lanes == 4) {
+            NBX_REQUIRE(sp.cbytes != 0, "the 4-lane spline kernel needs a coefficient array "
+                                        "under 4 GiB (set 1 or 2 lanes per particle)");
+            return launch_solve<Spline3<R, 4>, R, 4>(regroup<4>(sp), src, a, cfg, s);
+        }
         if (cfg.lanes == 2) return launch_solve<Spline3<R, 2>, R, 2>(regroup<2>(sp), src, a, cfg, s);
         return launch_solve<Spline3<R, 1>, R, 1>(sp, src, a, cfg, s);
     }

@@ -310,6 +310,9 @@ struct Spline3 {
     static constexpr int kRows = sizeof(R) == sizeof(double) ? 16 : 8;
     mutable P cst[kReuse ? kRows : 1];
     mutable long ckey = -1;
+    // bytes of C (the 4-lane form's buffer loads: under 4 GiB, checked by
+    // the dispatcher)
+    unsigned cbytes = 0;
 
     __device__ __forceinline__ void cell(int d, double q, int& i, R& s) const {
         const double u = (q - lo[d]) * dinv[d];
@@ -423,13 +426,26 @@ struct Spline3 {
                 const long key = static_cast<long>(it) * st + static_cast<long>(ix) * sx + iy;
                 if (key != ckey) {
                     ckey = key;
-                    const P* lb = base + g;
+                    // buffer loads: the lane's origin is the vector offset,
+                    // each row's offset a scalar one (no per-load address
+                    // arithmetic, and no 64-bit row pointers held in
+                    // scalar registers)
+                    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                        const_cast<P*>(C), 0, static_cast<int>(cbytes), 0x00020000);
+                    // (time slices step the vector offset, x rows are the
+                    // scalar offsets: three live scalars, not sixteen)
+                    const int vo = static_cast<int>((key + g) * static_cast<long>(sizeof(P)));
+                    const int tstep = (sizeof(R) == sizeof(double) ? 1 : 2) * st *
+                                      static_cast<int>(sizeof(P));
+                    const int xstep = sx * static_cast<int>(sizeof(P));
 #pragma unroll
                     for (int a = 0; a < kRows / 4; ++a)
 #pragma unroll
-                        for (int b = 0; b < 4; ++b)
-                            cst[a * 4 + b] = lb[(sizeof(R) == sizeof(double) ? a : 2 * a) * st +
-                                                b * sx];
+                        for (int b = 0; b < 4; ++b) {
+                            const auto raw = __builtin_amdgcn_raw_buffer_load_b128(
+                                rsrc, vo + a * tstep, b * xstep, 0);
+                            __builtin_memcpy(&cst[a * 4 + b], &raw, sizeof(P));
+                        }
                 }
                 // the same sums as column_sums over the held rows
 #pragma unroll
