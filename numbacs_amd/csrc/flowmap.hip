@@ -283,6 +283,7 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap(Flow f, 
                                                  double2* __restrict__ out,
                                                  int32_t* __restrict__ status,
                                                  int32_t* __restrict__ nsteps) {
+    f.prepare();
     const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     int64_t p = tid / G;
     if (p >= P) return;  // whole groups leave together (P counts slots for grids)
@@ -357,6 +358,7 @@ __global__ __launch_bounds__(64) void k_flowmap_staged(
     int32_t* __restrict__ status, int32_t* __restrict__ nsteps) {
     using Pt = typename CoefPair<R>::type;
     __shared__ Pt stage[kStageElems];
+    f.prepare();
     const int lane = static_cast<int>(threadIdx.x);
     const int64_t slot = static_cast<int64_t>(blockIdx.x) * 16 + (lane >> 2);
     const int64_t p = slot < P ? src.particle(slot) : -1;
@@ -475,6 +477,7 @@ __global__ __launch_bounds__(64) void k_flowmap_staged(
 template <class Flow, class R>
 __global__ void k_flow_eval(Flow f, const double* __restrict__ t, const double2* __restrict__ pts,
                             int64_t n, double2* __restrict__ out) {
+    f.prepare();
     const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (k >= n) return;
     R a, b;

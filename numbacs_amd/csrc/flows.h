@@ -21,6 +21,7 @@ constexpr double kPi = 3.141592653589793;
 // The float form evaluates the same expression in float.
 template <class R>
 struct DoubleGyre {
+    __device__ __forceinline__ void prepare() {}
     double p0, A, eps, alpha, omega, psi;
 
     __device__ __forceinline__ void operator()(double t, double y0d, double y1d, R& f0,
@@ -62,6 +63,7 @@ struct DoubleGyre {
 // p = [dir, U0, L, A1, A2, A3, k1, k2, k3, c1, c2, c3].
 template <class R>
 struct BickleyJet {
+    __device__ __forceinline__ void prepare() {}
     double p[12];
 
     __device__ __forceinline__ void operator()(double t, double y0, double y1, R& f0,
@@ -273,6 +275,17 @@ __device__ __forceinline__ double group_sum(double v) {
 // kernels built without it (L = false) serve modes 0 and 2 with fewer
 // registers (FP32 4-lane flow map: 167 instead of 234 VGPRs, 3 waves per
 // SIMD instead of 2; FP64: 221 instead of 256).
+// A uniform double moved into a vector register pair.  The flow kernels hold
+// the spline's box and spacing for their whole solve; in scalar registers
+// (next to the solver's constants and the kernel's pointers) they spill to
+// vector-register lanes and every evaluation reloads them one lane read at
+// a time.  The copy cannot be folded back into the scalar value.
+__device__ __forceinline__ double in_vgpr(double x) {
+    double y;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(y) : "v"(x));
+    return y;
+}
+
 // A box of coefficient elements copied to LDS by a one-wavefront block for
 // the stencils of one DOP853 step (cells it0.., ix0.., iy0.. of the
 // coefficient array, bt x bx x by elements, y fastest): a stencil whose 4 x 4
@@ -313,6 +326,16 @@ struct Spline3 {
     // bytes of C (the 4-lane form's buffer loads: under 4 GiB, checked by
     // the dispatcher)
     unsigned cbytes = 0;
+
+    // at the start of a kernel, on its own copy of the functor
+    __device__ __forceinline__ void prepare() {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = in_vgpr(lo[d]);
+            hi[d] = in_vgpr(hi[d]);
+            dinv[d] = in_vgpr(dinv[d]);
+        }
+    }
 
     __device__ __forceinline__ void cell(int d, double q, int& i, R& s) const {
         const double u = (q - lo[d]) * dinv[d];
@@ -622,6 +645,7 @@ struct Spline3 {
 // bottleneck), rounded to R at the end.
 template <class R>
 struct Linear3 {
+    __device__ __forceinline__ void prepare() {}
     const double2* __restrict__ V;
     double lo[3], hi[3], step[3];
     int n[3];

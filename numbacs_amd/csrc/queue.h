@@ -190,6 +190,8 @@ __global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : NBX_FLOWMAP_MIN_WAVES) vo
     int nreq, ChunkQueue q, Dop853State<R>* __restrict__ state, double rtol, double atol,
     long long nmax) {
     constexpr int kSlots = 64 / G;
+    // (no f.prepare(): this kernel's solver state, tile bookkeeping and
+    // handoff leave no vector registers for the box and spacing)
     const int lane = static_cast<int>(__lane_id());
     const bool lead = (lane & (G - 1)) == 0;
     for (;;) {
@@ -264,6 +266,7 @@ template <class Flow, class R, int G>
 __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap_tiles(
     Flow f, const ReqDev* __restrict__ table, const int64_t* __restrict__ slot_off, int nreq,
     int64_t total_slots, double rtol, double atol, long long nmax) {
+    f.prepare();
     const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     const int64_t q = tid / G;
     if (q >= total_slots) return;  // whole groups leave together
