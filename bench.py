@@ -62,8 +62,8 @@ def parse():
     ap.add_argument("--workload", default="gridded",
                     choices=["gridded", "double_gyre", "timeseries", "precision", "surrogate",
                              "composition", "next", "copycal", "stencil"])
-    ap.add_argument("--surrogate-samples", type=int, default=4_000_000)
-    ap.add_argument("--surrogate-t0s", type=int, default=2)
+    ap.add_argument("--surrogate-samples", type=int, default=20_000_000)
+    ap.add_argument("--surrogate-t0s", type=int, default=96)
     return ap.parse_args()
 
 
@@ -211,7 +211,7 @@ def bench_gridded(args):
             "gather": {"bytes_per_launch": bytes_launch,
                        "achieved_GBs": achieved_gbs,
                        "unit": "12,288 B of spline coefficients per attempted step, served by "
-                               "L1/L2 (not an HBM figure)"},
+                               "the lanes' stencil registers, L1 and L2 (not an HBM figure)"},
         },
         "particle_steps_per_s": steps_total / (fm_ms * 1e-3),
         "mean_steps_per_particle": steps_total / P,

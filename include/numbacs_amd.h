@@ -152,13 +152,17 @@ int nbx_flowmap_compose(const double* flowmaps, int nslots, int first, int nT, d
  * lanes_per_particle (1, 2 or 4; spline flows only: the lanes of a group
  * split each tricubic gather; spline flows created with linear
  * extrapolation always run one lane per particle, in the kernel that
- * carries the gradient path), threads per block (64, 128 or 256), and
- * persistent_blocks_per_cu (0: one lane group per particle in one launch;
- * n > 0: a persistent grid of n blocks per compute unit pulling particles
- * from a device work queue; grid flow maps and batches only).
- * Process-wide; set by the autotuner (numbacs_amd.autotune).  No reference
- * counterpart (the reference's parallel loop has no tunables,
- * integration.py:166). */
+ * carries the gradient path; 4 lanes need a coefficient array under
+ * 4 GiB), threads per block (64, 128 or 256), and persistent_blocks_per_cu
+ * (0: one lane group per particle in one launch over the particle tiles;
+ * n > 0: a persistent grid of n blocks per compute unit taking wavefront
+ * tiles from a device queue, a tile going back to the queue every 16
+ * steps while a particle of it is still running; -1: spline coefficients
+ * staged through LDS per step, spline flows at 4 lanes and 64-thread
+ * blocks; persistent and staged launches serve grid flow maps and batches,
+ * other sources run the plain launch).  Process-wide; set by the autotuner
+ * (numbacs_amd.autotune).  No reference counterpart (the reference's
+ * parallel loop has no tunables, integration.py:166). */
 int nbx_set_launch_config(int kind, int precision, int lanes_per_particle, int block_threads,
                           int persistent_blocks_per_cu);
 int nbx_get_launch_config(int kind, int precision, int* lanes_per_particle, int* block_threads,
