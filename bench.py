@@ -191,7 +191,9 @@ def bench_gridded(args):
             "field_sha256": W["sha"],
             "parallelism": "single GPU",
             "launch": {"lanes_per_particle": tuned[0], "block": tuned[1],
-                       "persistent_blocks_per_cu": tuned[2]},
+                       "persistent_blocks_per_cu": tuned[2],
+                       "tile_order": ("edge-first" if autotune.get_order(_lib.NBX_FP64)
+                                      else "plain") if tuned[2] == 0 else None},
         },
         "roofline": {
             "bound": "fp64",
@@ -300,7 +302,7 @@ def timeseries_legs(reps=3, tune=True, oracle_rows=16):
         if tune:
             cands = [(4, blk, per) for blk in (64, 128, 256) for per in (0, 2, 4, 8)]
             cfg, _ = autotune.tune(_lib.NBX_FLOW_SPLINE3, pc, b.run, P, reps=1,
-                                   use_cache=False, cands=cands)
+                                   use_cache=False, cands=cands, order=False)
         else:
             cfg = autotune.get(_lib.NBX_FLOW_SPLINE3, pc)
         b.capture()

@@ -50,6 +50,8 @@ SIGNATURES = {
                                    _c_p, _c_p]),
     "nbx_set_launch_config": (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i]),
     "nbx_get_launch_config": (_c_i, [_c_i, _c_i, _c_p, _c_p, _c_p]),
+    "nbx_set_tile_order": (_c_i, [_c_i, _c_i]),
+    "nbx_get_tile_order": (_c_i, [_c_i, _c_p]),
     "nbx_batch_create": (_c_i, [_c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_i64, _c_i, _c_p]),
     "nbx_batch_particles": (_c_i64, [_c_p]),
     "nbx_batch_run": (_c_i, [_c_p, _c_p]),

@@ -40,6 +40,17 @@ def set_config(kind, precision, cfg):
     _lib.check(_lib.load().nbx_set_launch_config(kind, precision, *map(int, cfg)))
 
 
+def get_order(precision):
+    """Tile launch order of the spline grid launch: 1 edge-first, 0 plain."""
+    v = ctypes.c_int()
+    _lib.check(_lib.load().nbx_get_tile_order(precision, ctypes.byref(v)))
+    return v.value
+
+
+def set_order(precision, edge_first):
+    _lib.check(_lib.load().nbx_set_tile_order(precision, int(edge_first)))
+
+
 def _key(kind, precision, size):
     t = _lib.torch()
     return f"{t.cuda.get_device_name()}|{kind}|{precision}|{size}"
@@ -53,9 +64,12 @@ def _load_cache():
         return {}
 
 
-def tune(kind, precision, run, size, reps=3, persistent=True, use_cache=True, cands=None):
+def tune(kind, precision, run, size, reps=3, persistent=True, use_cache=True, cands=None,
+         order=True):
     """run() enqueues one flow map with the installed launch shape; cands
-    (optional) restricts the candidate shapes.  Returns (best_cfg,
+    (optional) restricts the candidate shapes; order also times the grid
+    launch's tile order (edge-first against plain) when the best shape is
+    the spline grid launch (run() must be that launch).  Returns (best_cfg,
     {cfg: ms})."""
     t = _lib.torch()
     key = _key(kind, precision, size)
@@ -63,6 +77,8 @@ def tune(kind, precision, run, size, reps=3, persistent=True, use_cache=True, ca
     if key in cache:
         best = tuple(cache[key]["cfg"])
         set_config(kind, precision, best)
+        if "edge_first" in cache[key]:
+            set_order(precision, cache[key]["edge_first"])
         return best, cache[key]["times"]
     old = get(kind, precision)
     times = {}
@@ -79,6 +95,23 @@ def tune(kind, precision, run, size, reps=3, persistent=True, use_cache=True, ca
     best = tuple(int(v) for v in best_s.split(","))
     set_config(kind, precision, best if best else old)
     cache[key] = {"cfg": list(best), "times": times}
+    if order and kind == _lib.NBX_FLOW_SPLINE3 and best[2] == 0:
+        # the grid launch's tile order: edge-first against plain
+        order_ms = {}
+        for edge_first in (0, 1):
+            set_order(precision, edge_first)
+            run()
+            t.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run()
+            t.cuda.synchronize()
+            order_ms[edge_first] = 1e3 * (time.perf_counter() - t0) / reps
+        pick = min(order_ms, key=order_ms.get)
+        set_order(precision, pick)
+        times[best_s + (",edge" if pick else ",plain")] = order_ms[pick]
+        times[best_s + (",plain" if pick else ",edge")] = order_ms[1 - pick]
+        cache[key]["edge_first"] = pick
     try:
         os.makedirs(os.path.dirname(CACHE), exist_ok=True)
         with open(CACHE, "w") as fh:

@@ -70,6 +70,13 @@ def test_batch_and_launch_config_validation_without_gpu():
     g, b, q = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     assert lib.nbx_get_launch_config(2, 1, ctypes.byref(g), ctypes.byref(b), ctypes.byref(q)) == 0
     assert g.value in (1, 2, 4) and b.value in (64, 128, 256) and q.value >= 0
+    assert lib.nbx_set_tile_order(2, 1) != 0 and b"precision" in lib.nbx_last_error()
+    assert lib.nbx_set_tile_order(0, 2) != 0 and b"edge_first" in lib.nbx_last_error()
+    e = ctypes.c_int(-1)
+    assert lib.nbx_get_tile_order(0, ctypes.byref(e)) == 0 and e.value in (0, 1)
+    assert lib.nbx_set_tile_order(1, 0) == 0
+    assert lib.nbx_get_tile_order(1, ctypes.byref(e)) == 0 and e.value == 0
+    assert lib.nbx_set_tile_order(1, 1) == 0
     assert lib.nbx_mlp_gemm(0, None, None, 65, 64, 32, None, None, None, None, None) != 0
     assert b"multiples" in lib.nbx_last_error()
     lib.nbx_flow_destroy(flow)

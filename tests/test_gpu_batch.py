@@ -30,9 +30,12 @@ def restore_config(lib):
     from numbacs_amd import autotune
 
     saved = {(k, p): autotune.get(k, p) for k in (0, 1, 2) for p in (0, 1)}
+    orders = {p: autotune.get_order(p) for p in (0, 1)}
     yield autotune
     for (k, p), cfg in saved.items():
         autotune.set_config(k, p, cfg)
+    for p, o in orders.items():
+        autotune.set_order(p, o)
 
 
 def _dg_requests():
@@ -183,15 +186,16 @@ def test_fp32_displacement_error_double_gyre(lib):
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("case", ["gridded", "wrap_nearest_mask"])
 def test_staged_and_queue_launches_equal_grid_launch(lib, restore_config, precision, case):
-    """The LDS-staged launch (coefficient boxes copied to LDS per step,
-    persistent -1) and the persistent tile queue (tiles handed back every
-    16 steps with their solver states stored, persistent n > 0) run the same
-    arithmetic as the grid launch: equal flow maps, status and step counts,
-    on the gridded workload and on a global field whose boxes straddle the
-    longitude wrap, with nearest extrapolation and a mask."""
+    """The grid launch in edge-first tile order (tiles nearest the velocity
+    box's edge launched first) and in plain tile order, the LDS-staged
+    launch (coefficient boxes copied to LDS per step, persistent -1) and the
+    persistent tile queue (tiles handed back every 16 steps with their
+    solver states stored, persistent n > 0) run the same arithmetic: equal
+    flow maps, status and step counts, on the gridded workload and on a
+    global field whose boxes straddle the longitude wrap, with nearest
+    extrapolation and a mask."""
     from numbacs_amd import autotune
     from numbacs_amd.flows import get_flow_2D, get_interp_arrays_2D
-    from numbacs_amd.integration import flowmap_grid_2d_device
     from numbacs_amd.workloads import particle_axes, synthetic_velocity
 
     if case == "gridded":
@@ -215,9 +219,22 @@ def test_staged_and_queue_launches_equal_grid_launch(lib, restore_config, precis
         t0, T = 40.0, -36.0
     xd, yd = lib.to_dev(x), lib.to_dev(y)
     md = None if mask is None else lib.to_dev(mask.astype(np.uint8), np.uint8)
+    # launches that must agree bit for bit: the 4-lane forms, then the
+    # 2-lane form (its own summation order) in both tile orders
+    groups = [(((4, 64, 0), 0), ((4, 64, 0), 1), ((4, 256, 0), 1), ((4, 64, -1), 1),
+               ((4, 64, 4), 1), ((4, 256, 2), 1)),
+              (((2, 128, 0), 0), ((2, 128, 0), 1))]
+    for group in groups:
+        _equal_launches(lib, autotune, precision, group, f, t0, T, xd, yd, md, x, y)
+
+
+def _equal_launches(lib, autotune, precision, group, f, t0, T, xd, yd, md, x, y):
+    from numbacs_amd.integration import flowmap_grid_2d_device
+
     res = []
-    for cfg in ((4, 64, 0), (4, 64, -1), (4, 64, 4), (4, 256, 2)):
+    for cfg, edge_first in group:
         autotune.set_config(lib.NBX_FLOW_SPLINE3, precision, cfg)
+        autotune.set_order(precision, edge_first)
         st = lib.empty((len(x), len(y)), "i32")
         ns = lib.empty((len(x), len(y)), "i32")
         fm = flowmap_grid_2d_device(f, t0, T, xd, yd, [-1.0], maskd=md, precision=precision,
