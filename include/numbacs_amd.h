@@ -361,6 +361,12 @@ int nbx_mlp_colsum(const uint16_t* da, int64_t n, int width, float* gb, nbx_stre
 int nbx_mlp_adam(float* w, const float* g, float* m, float* v, uint16_t* w16, int64_t n,
                  float lr, float b1, float b2, float eps, int step, float weight_decay,
                  nbx_stream stream);
+/* The same Adam step with the learning rate and the bias corrections in
+ * device memory: sched = [lr, 1 - b1^t, 1 - b2^t] (3 floats), so a
+ * captured training step replays for every step of a schedule. */
+int nbx_mlp_adam_sched(float* w, const float* g, float* m, float* v, uint16_t* w16, int64_t n,
+                       const float* sched, float b1, float b2, float eps, float weight_decay,
+                       nbx_stream stream);
 int nbx_mlp_to_bf16(const float* x, int64_t n, uint16_t* y, nbx_stream stream);
 
 /* The trained network for fused inference: feature shift/scale (4 each) and

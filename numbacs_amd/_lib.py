@@ -67,6 +67,8 @@ SIGNATURES = {
     "nbx_mlp_colsum": (_c_i, [_c_p, _c_i64, _c_i, _c_p, _c_p]),
     "nbx_mlp_adam": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, ctypes.c_float, ctypes.c_float,
                             ctypes.c_float, ctypes.c_float, _c_i, ctypes.c_float, _c_p]),
+    "nbx_mlp_adam_sched": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, _c_p]),
     "nbx_mlp_to_bf16": (_c_i, [_c_p, _c_i64, _c_p, _c_p]),
     "nbx_mlp_infer": (_c_i, [_c_p, _c_p, _c_i64, _c_p, _c_p]),
     "nbx_ftle_grid_2d": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _c_d, _c_p, _c_p, _c_p]),

@@ -59,18 +59,22 @@ __device__ __forceinline__ float gelu_grad(float a) {
     return 0.5f * (1.0f + t) + 0.5f * a * (1.0f - t * t) * dinner;
 }
 
-enum Epilogue { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_GRAD = 2 };
+// EPI_F32_ADD: split-K partial sums added into C32 (zeroed beforehand)
+enum Epilogue { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_GRAD = 2, EPI_F32_ADD = 3 };
 
 // C(m, n) = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak],
 // B(k, n) = B[k*sbk + n*sbn].  AK: sak == 1 (else sam == 1); BKC: sbk == 1
-// (else sbn == 1).  M, N multiples of 64, K a multiple of 32.
+// (else sbn == 1).  M, N multiples of 64, K a multiple of 32.  Block z
+// covers k in [z*ksplit, min((z+1)*ksplit, K)) (ksplit a multiple of 32;
+// ksplit = K without a split).
 template <bool AK, bool BKC, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, long sam, long sak,
                                               const uint16_t* __restrict__ B, long sbk, long sbn,
                                               int M, int N, int K, float* __restrict__ C32,
                                               long ldc, uint16_t* __restrict__ C16, long ldc16,
                                               const float* __restrict__ bias,
-                                              const float* __restrict__ pre, long ldp) {
+                                              const float* __restrict__ pre, long ldp,
+                                              int ksplit) {
     __shared__ __attribute__((aligned(16))) uint16_t As[BM][LDK];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][LDK];
     const int t = threadIdx.x;
@@ -83,7 +87,9 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, lo
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < K; k0 += BK) {
+    const int kb = static_cast<int>(blockIdx.z) * ksplit;
+    const int ke = kb + ksplit < K ? kb + ksplit : K;
+    for (int k0 = kb; k0 < ke; k0 += BK) {
         // ---- stage A (64 x 32) and B (32 x 64) with k contiguous ----
         if (AK) {
             const int row = t >> 2, kq = (t & 3) * 8;
@@ -133,6 +139,8 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, lo
                 const float v = acc[i][j][q];
                 if (EPI == EPI_F32) {
                     C32[m * ldc + n] = v;
+                } else if (EPI == EPI_F32_ADD) {
+                    atomicAdd(&C32[m * ldc + n], v);
                 } else if (EPI == EPI_BIAS_GELU) {
                     const float a = v + bias[n];
                     if (C32) C32[m * ldc + n] = a;
@@ -173,10 +181,16 @@ __global__ void k_out_fwd(const uint16_t* __restrict__ h, int64_t n, int width,
     if (r >= n) return;
     float s0 = 0.f, s1 = 0.f;
     const uint16_t* hr = h + r * width;
-    for (int k = 0; k < width; ++k) {
-        const float a = bf2f(hr[k]);
-        s0 += a * bf2f(W[2 * k]);
-        s1 += a * bf2f(W[2 * k + 1]);
+    // 16-byte loads of the row (width a multiple of 8), sums in k order
+    for (int k8 = 0; k8 < width; k8 += 8) {
+        const u16x8 hv = *reinterpret_cast<const u16x8*>(hr + k8);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k8 + q;
+            const float a = bf2f(hv[q]);
+            s0 += a * bf2f(W[2 * k]);
+            s1 += a * bf2f(W[2 * k + 1]);
+        }
     }
     y[2 * r] = s0 + b[0];
     y[2 * r + 1] = s1 + b[1];
@@ -215,13 +229,17 @@ __global__ void k_out_bwd_data(const float* __restrict__ dy, int64_t n, int widt
     da[idx] = f2bf(dh * gelu_grad(pre[idx]));
 }
 
+// Rows per block of the column-sum kernels (k_out_bwd_weight, k_colsum):
+// enough blocks to fill the GPU at a training batch of a few thousand rows.
+constexpr int kSumRows = 32;
+
 // gW_out[k][c] += sum_r h[r][k] dy[r][c], gb_out[c] += sum_r dy[r][c]:
-// one block per 256-row chunk, one thread per k, atomics across chunks.
+// one block per kSumRows-row chunk, one thread per k, atomics across chunks.
 __global__ void k_out_bwd_weight(const uint16_t* __restrict__ h, const float* __restrict__ dy,
                                  int64_t n, int width, float* __restrict__ gW,
                                  float* __restrict__ gb) {
-    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 256;
-    const int64_t r1 = r0 + 256 < n ? r0 + 256 : n;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSumRows;
+    const int64_t r1 = r0 + kSumRows < n ? r0 + kSumRows : n;
     for (int k = threadIdx.x; k < width; k += blockDim.x) {
         float s0 = 0.f, s1 = 0.f;
         for (int64_t r = r0; r < r1; ++r) {
@@ -239,11 +257,11 @@ __global__ void k_out_bwd_weight(const uint16_t* __restrict__ h, const float* __
     }
 }
 
-// gb[c] = sum_r da[r][c] (bf16 rows), one block per 256-row chunk.
+// gb[c] = sum_r da[r][c] (bf16 rows), one block per kSumRows-row chunk.
 __global__ void k_colsum(const uint16_t* __restrict__ da, int64_t n, int width,
                          float* __restrict__ gb) {
-    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 256;
-    const int64_t r1 = r0 + 256 < n ? r0 + 256 : n;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSumRows;
+    const int64_t r1 = r0 + kSumRows < n ? r0 + kSumRows : n;
     for (int c = threadIdx.x; c < width; c += blockDim.x) {
         float s = 0.f;
         for (int64_t r = r0; r < r1; ++r) s += bf2f(da[r * width + c]);
@@ -262,6 +280,28 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
                        float b1, float b2, float eps, float c1, float c2, float wd) {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float upd = (mi / c1) / (sqrtf(vi / c2) + eps) + wd * w[i];
+    const float wi = w[i] - lr * upd;
+    w[i] = wi;
+    if (w16) w16[i] = f2bf(wi);
+}
+
+// Adam with the learning rate and both bias corrections read from device
+// memory (sched = [lr, 1 - b1^t, 1 - b2^t]), so one captured training step
+// can be replayed for every step of a schedule.
+__global__ void k_adam_sched(float* __restrict__ w, const float* __restrict__ g,
+                             float* __restrict__ m, float* __restrict__ v,
+                             uint16_t* __restrict__ w16, int64_t n,
+                             const float* __restrict__ sched, float b1, float b2, float eps,
+                             float wd) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const float lr = sched[0], c1 = sched[1], c2 = sched[2];
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -419,23 +459,42 @@ int nbx_mlp_gemm(int mode, const uint16_t* A, const uint16_t* B, int M, int N, i
         case 0:  // forward: C16 = gelu(A (MxK) @ B (KxN) + bias), C32 = pre-activation
             NBX_REQUIRE(C16 != nullptr && bias != nullptr, "forward needs C16 and bias");
             k_gemm<true, false, EPI_BIAS_GELU><<<grid, 256, 0, s>>>(A, K, 1, B, N, 1, M, N, K, C32,
-                                                                    N, C16, N, bias, nullptr, 0);
+                                                                    N, C16, N, bias, nullptr, 0,
+                                                                    K);
             break;
         case 1:  // backward data: C16 = bf16(bf16(A (MxK) @ B^T) * gelu'(pre)), B stored (N x K)
             NBX_REQUIRE(C16 != nullptr && pre != nullptr, "backward-data needs C16 and pre");
             k_gemm<true, true, EPI_GELU_GRAD><<<grid, 256, 0, s>>>(A, K, 1, B, 1, K, M, N, K,
                                                                    nullptr, 0, C16, N, nullptr,
-                                                                   pre, N);
+                                                                   pre, N, K);
             break;
-        case 2:  // backward weight: C32 = A^T @ B with A stored (K x M), B stored (K x N)
+        case 2: {  // backward weight: C32 = A^T @ B with A stored (K x M), B stored (K x N)
             NBX_REQUIRE(C32 != nullptr, "backward-weight needs C32");
-            k_gemm<false, false, EPI_F32><<<grid, 256, 0, s>>>(A, 1, M, B, N, 1, M, N, K, C32, N,
-                                                               nullptr, 0, nullptr, nullptr, 0);
+            // K is the batch: split it over blocks when the output has few
+            // tiles (a 256 x 256 weight is 16 tiles on 256 CUs), each block
+            // adding its partial tile into C32
+            const int tiles = (M / BM) * (N / BN);
+            int splits = 256 / tiles;
+            if (splits > K / 256) splits = K / 256;
+            if (splits <= 1) {
+                k_gemm<false, false, EPI_F32><<<grid, 256, 0, s>>>(A, 1, M, B, N, 1, M, N, K, C32,
+                                                                   N, nullptr, 0, nullptr, nullptr,
+                                                                   0, K);
+            } else {
+                const int ksplit = (K / splits + BK - 1) / BK * BK;
+                const dim3 g3(N / BN, M / BM, (K + ksplit - 1) / ksplit);
+                k_fill_f32<<<blocks_for(static_cast<int64_t>(M) * N, 256), 256, 0, s>>>(
+                    C32, static_cast<int64_t>(M) * N, 0.f);
+                k_gemm<false, false, EPI_F32_ADD><<<g3, 256, 0, s>>>(A, 1, M, B, N, 1, M, N, K,
+                                                                     C32, N, nullptr, 0, nullptr,
+                                                                     nullptr, 0, ksplit);
+            }
             break;
+        }
         case 3:  // plain: C32 = A (MxK) @ B (KxN)
             NBX_REQUIRE(C32 != nullptr, "plain GEMM needs C32");
             k_gemm<true, false, EPI_F32><<<grid, 256, 0, s>>>(A, K, 1, B, N, 1, M, N, K, C32, N,
-                                                              nullptr, 0, nullptr, nullptr, 0);
+                                                              nullptr, 0, nullptr, nullptr, 0, K);
             break;
         default:
             set_error("unknown GEMM mode");
@@ -480,7 +539,8 @@ int nbx_mlp_out_backward(const float* dy, const uint16_t* h, const float* pre, i
     hipStream_t s = as_stream(stream);
     k_fill_f32<<<blocks_for(2 * width, 256), 256, 0, s>>>(gW, 2 * width, 0.f);
     k_fill_f32<<<1, 2, 0, s>>>(gb, 2, 0.f);
-    k_out_bwd_weight<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(h, dy, n, width, gW, gb);
+    k_out_bwd_weight<<<static_cast<unsigned>((n + kSumRows - 1) / kSumRows), 256, 0, s>>>(
+        h, dy, n, width, gW, gb);
     k_out_bwd_data<<<blocks_for(n * width, 256), 256, 0, s>>>(dy, n, width, W, pre, da);
     NBX_LAUNCHED("mlp_out_bwd");
     return 0;
@@ -490,7 +550,8 @@ int nbx_mlp_colsum(const uint16_t* da, int64_t n, int width, float* gb, nbx_stre
     NBX_REQUIRE(n > 0, "empty batch");
     hipStream_t s = as_stream(stream);
     k_fill_f32<<<blocks_for(width, 256), 256, 0, s>>>(gb, width, 0.f);
-    k_colsum<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(da, n, width, gb);
+    k_colsum<<<static_cast<unsigned>((n + kSumRows - 1) / kSumRows), 256, 0, s>>>(da, n, width,
+                                                                                  gb);
     NBX_LAUNCHED("mlp_colsum");
     return 0;
 }
@@ -504,6 +565,17 @@ int nbx_mlp_adam(float* w, const float* g, float* m, float* v, uint16_t* w16, in
     k_adam<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(w, g, m, v, w16, n, lr, b1, b2, eps,
                                                               c1, c2, weight_decay);
     NBX_LAUNCHED("mlp_adam");
+    return 0;
+}
+
+int nbx_mlp_adam_sched(float* w, const float* g, float* m, float* v, uint16_t* w16, int64_t n,
+                       const float* sched, float b1, float b2, float eps, float weight_decay,
+                       nbx_stream stream) {
+    NBX_REQUIRE(sched != nullptr, "NULL schedule");
+    if (n == 0) return 0;
+    k_adam_sched<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(w, g, m, v, w16, n, sched,
+                                                                    b1, b2, eps, weight_decay);
+    NBX_LAUNCHED("mlp_adam_sched");
     return 0;
 }
 

@@ -125,6 +125,22 @@ class MLPSurrogate:
                 da = da_prev
         return loss, gW, gb
 
+    def train_step_sched(self, inp, target, sched, b1=0.9, b2=0.999, eps=1e-8,
+                         weight_decay=0.0):
+        """One Adam step whose learning rate and bias corrections come from
+        the device tensor sched = [lr, 1 - b1^t, 1 - b2^t]: every launch
+        reads only device memory, so the step can be captured once and
+        replayed for a whole schedule."""
+        L = _lib.load()
+        s = _lib.stream_ptr()
+        loss, gW, gb = self.gradients(inp, target)
+        for k, (p, g, m, v) in enumerate(zip(self.params, gW + gb, self.m, self.v)):
+            w16 = self.W16[k] if k < len(self.W) else None
+            _lib.check(L.nbx_mlp_adam_sched(_lib.ptr(p), _lib.ptr(g), _lib.ptr(m), _lib.ptr(v),
+                                            _lib.ptr(w16), p.numel(), _lib.ptr(sched), b1, b2,
+                                            eps, weight_decay, s))
+        return loss
+
     def train_step(self, inp, target, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, weight_decay=0.0):
         L = _lib.load()
         s = _lib.stream_ptr()
@@ -234,29 +250,76 @@ def sample_requests(nreq, per_axis, box, t0_range, T_range, seed=0):
     return out
 
 
+def adam_schedule(steps, lr, lr_min, b1=0.9, b2=0.999):
+    """Per step k (from 0): [lr_k, 1 - b1^(k+1), 1 - b2^(k+1)] in float32, with
+    the cosine learning-rate decay from lr to lr_min."""
+    k = np.arange(steps, dtype=np.float64)
+    lrs = lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * k / max(steps, 1)))
+    t = np.arange(1, steps + 1, dtype=np.float32)
+    c1 = np.float32(1) - np.power(np.float32(b1), t)
+    c2 = np.float32(1) - np.power(np.float32(b2), t)
+    return np.stack([lrs.astype(np.float32), c1, c2], axis=1)
+
+
 def train(net, inputs, targets, epochs=3, batch=8192, lr=2e-3, lr_min=1e-4, seed=0,
-          log=None):
+          log=None, graph=True):
     """Adam over shuffled mini-batches with a cosine learning-rate decay;
-    targets in physical units (divided by net.out_scale here)."""
+    targets in physical units (divided by net.out_scale here).
+
+    With graph (the default) one training step -- the batch gather, the
+    forward and backward passes and Adam, about 30 launches -- is captured
+    once into a HIP graph and replayed; per step the host only copies the
+    batch's indices and its schedule row.  Without it every launch is
+    issued from Python.  The loss and bias-gradient sums add with float
+    atomics, so two runs agree to rounding, not bit for bit."""
     t = _lib.torch()
     n = int(inputs.shape[0]) // batch * batch
     tgt = (targets / net.out_scale).contiguous()
     g = t.Generator(device=inputs.device)
     g.manual_seed(seed)
     steps = epochs * (n // batch)
-    k = 0
+    table = t.from_numpy(adam_schedule(steps, lr, lr_min)).to(inputs.device)
     losses = []
+    if not graph:
+        k = 0
+        for ep in range(epochs):
+            perm = t.randperm(int(inputs.shape[0]), device=inputs.device, generator=g)[:n]
+            for s in range(n // batch):
+                idx = perm[s * batch:(s + 1) * batch]
+                loss = net.train_step_sched(inputs[idx].contiguous(), tgt[idx].contiguous(),
+                                            table[k])
+                k += 1
+                net.step += 1
+                if s % 200 == 0:
+                    losses.append(float(loss.item()))
+                    if log:
+                        log(f"epoch {ep} step {s} loss {losses[-1]:.3e}")
+        return losses
+    dev = inputs.device
+    idx = t.zeros(batch, dtype=t.long, device=dev)
+    inp_s = t.empty((batch, int(inputs.shape[1])), dtype=inputs.dtype, device=dev)
+    tgt_s = t.empty((batch, int(tgt.shape[1])), dtype=tgt.dtype, device=dev)
+    sched = table[0].clone()
+    cg = t.cuda.CUDAGraph()
+    with t.cuda.graph(cg):
+        t.index_select(inputs, 0, idx, out=inp_s)
+        t.index_select(tgt, 0, idx, out=tgt_s)
+        loss = net.train_step_sched(inp_s, tgt_s, sched)
+    k = 0
     for ep in range(epochs):
-        perm = t.randperm(int(inputs.shape[0]), device=inputs.device, generator=g)[:n]
+        perm = t.randperm(int(inputs.shape[0]), device=dev, generator=g)[:n]
         for s in range(n // batch):
-            idx = perm[s * batch:(s + 1) * batch]
-            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * k / max(steps, 1)))
-            loss = net.train_step(inputs[idx].contiguous(), tgt[idx].contiguous(), lr=cur)
+            idx.copy_(perm[s * batch:(s + 1) * batch])
+            sched.copy_(table[k])
+            cg.replay()
             k += 1
             if s % 200 == 0:
                 losses.append(float(loss.item()))
                 if log:
                     log(f"epoch {ep} step {s} loss {losses[-1]:.3e}")
+    t.cuda.synchronize()
+    del cg
+    net.step += steps
     return losses
 
 

@@ -149,6 +149,35 @@ def test_training_reduces_loss(lib):
     assert last < 0.1 * first, (first, last)
 
 
+def test_graph_training_matches_eager(lib):
+    """Training with the step captured once and replayed (the default) and
+    with every launch issued eagerly follow the same schedule: both fit a
+    smooth field by 5x, to losses within a factor 2 of each other (the
+    loss and bias-gradient sums add with float atomics, so the two runs
+    agree to rounding, not bit for bit), and the replayed run leaves the
+    network's step count where the eager one does."""
+    import torch as t
+
+    from numbacs_amd import surrogate as S
+
+    res = {}
+    for graph in (True, False):
+        net, inp, _ = _net(lib, seed=5, n=8192)
+        u = (inp - net.state()["shift"]) / net.state()["scale"]
+        tgt = np.column_stack([np.sin(2 * u[:, 0]) * np.cos(u[:, 1]) + 0.3 * u[:, 3],
+                               np.cos(u[:, 0] + u[:, 2]) * 0.5]).astype(np.float32)
+        inp_d = t.from_numpy(inp).to("cuda")
+        tgt_d = t.from_numpy(tgt).to("cuda") * net.out_scale
+        first = float(net.gradients(inp_d, tgt_d / net.out_scale)[0].item())
+        S.train(net, inp_d, tgt_d, epochs=60, batch=2048, lr=2e-3, lr_min=2e-3, graph=graph)
+        last = float(net.gradients(inp_d, tgt_d / net.out_scale)[0].item())
+        res[graph] = (first, last, net.step)
+    for first, last, _ in res.values():
+        assert last < 0.2 * first, res
+    assert max(res[True][1], res[False][1]) < 2.0 * min(res[True][1], res[False][1]), res
+    assert res[True][2] == res[False][2] == 240
+
+
 @pytest.fixture(scope="module")
 def trained(lib):
     """A surrogate trained on DOP853 samples of a small gridded field."""
