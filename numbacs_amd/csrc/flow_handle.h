@@ -62,6 +62,7 @@ Spline3<R, 1, L> make_spline(const nbx_flow* fl, double p0) {
     s.extrap = fl->extrap;
     s.r = fl->r;
     s.p0 = p0;
+    s.kdeg = p0 * (180 / (kPi * fl->r));
     return s;
 }
 
@@ -101,6 +102,7 @@ Spline3<R, G, false, S> regroup(const Spline3<R, 1, false>& a) {
     s.extrap = a.extrap;
     s.r = a.r;
     s.p0 = a.p0;
+    s.kdeg = a.kdeg;
     return s;
 }
 

@@ -22,6 +22,7 @@ constexpr double kPi = 3.141592653589793;
 template <class R>
 struct DoubleGyre {
     __device__ __forceinline__ void prepare() {}
+    __device__ __forceinline__ void prepare_factors() {}
     double p0, A, eps, alpha, omega, psi;
 
     __device__ __forceinline__ void operator()(double t, double y0d, double y1d, R& f0,
@@ -64,6 +65,7 @@ struct DoubleGyre {
 template <class R>
 struct BickleyJet {
     __device__ __forceinline__ void prepare() {}
+    __device__ __forceinline__ void prepare_factors() {}
     double p[12];
 
     __device__ __forceinline__ void operator()(double t, double y0, double y1, R& f0,
@@ -315,6 +317,7 @@ struct Spline3 {
     int spherical, extrap;
     double r;     // sphere radius (km) for spherical = 1, 2
     double p0;    // integration direction
+    double kdeg;  // p0 * (180 / (pi r)): km/h -> degrees/h, set with p0 and r
     // 4-lane unstaged form: the lane's column of the last stencil it
     // gathered and that stencil's origin (element offset), so a query in
     // the same cell as the particle's previous one reads registers instead
@@ -335,6 +338,13 @@ struct Spline3 {
             hi[d] = in_vgpr(hi[d]);
             dinv[d] = in_vgpr(dinv[d]);
         }
+        prepare_factors();
+    }
+    // the direction and the degree factor only (for kernels with no
+    // vector registers to spare for the box)
+    __device__ __forceinline__ void prepare_factors() {
+        p0 = in_vgpr(p0);
+        kdeg = in_vgpr(kdeg);
     }
 
     __device__ __forceinline__ void cell(int d, double q, int& i, R& s) const {
@@ -615,18 +625,18 @@ struct Spline3 {
             f0 = static_cast<R>(p0 * static_cast<double>(su));
             f1 = static_cast<R>(p0 * static_cast<double>(sv));
         } else if constexpr (sizeof(R) == sizeof(double)) {
-            // km/h -> degrees/h with one factor p0 * 180 / (pi r) per flow
-            // (hoisted out of the solve), so the meridional component needs
-            // no division: within a few ulps of dividing each by pi r, and
-            // ~4% faster on the gridded flow map
-            const double k = p0 * (180 / (kPi * r));
-            f0 = (k * su) / cos_lat(deg_to_rad(y1));
-            f1 = k * sv;
+            // km/h -> degrees/h with one factor p0 * 180 / (pi r) per flow,
+            // computed on the host (left to the device, the compiler
+            // recomputed it, a division included, at every evaluation), so
+            // the meridional component needs no division: within a few ulps
+            // of dividing each by pi r
+            f0 = (kdeg * su) / cos_lat(deg_to_rad(y1));
+            f1 = kdeg * sv;
         } else {
             // the float form converts km/h to degrees/h in float too: the
             // velocity is already float, and the cosine of a float latitude
             // is good to ~1e-7, below the form's own rounding
-            const float k = static_cast<float>(p0 * (180 / (kPi * r)));
+            const float k = static_cast<float>(kdeg);
             f0 = (k * su) / cos_lat(static_cast<float>(y1) * static_cast<float>(kPi / 180));
             f1 = k * sv;
         }
@@ -646,6 +656,7 @@ struct Spline3 {
 template <class R>
 struct Linear3 {
     __device__ __forceinline__ void prepare() {}
+    __device__ __forceinline__ void prepare_factors() {}
     const double2* __restrict__ V;
     double lo[3], hi[3], step[3];
     int n[3];

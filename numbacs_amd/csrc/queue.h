@@ -190,8 +190,10 @@ __global__ __launch_bounds__(256, sizeof(R) == 4 ? 3 : NBX_FLOWMAP_MIN_WAVES) vo
     int nreq, ChunkQueue q, Dop853State<R>* __restrict__ state, double rtol, double atol,
     long long nmax) {
     constexpr int kSlots = 64 / G;
-    // (no f.prepare(): this kernel's solver state, tile bookkeeping and
-    // handoff leave no vector registers for the box and spacing)
+    // (only the factors, in FP64: this kernel's solver state, tile
+    // bookkeeping and handoff leave no vector registers for the box and
+    // spacing, and the FP32 instance none at its 3-wave budget)
+    if constexpr (sizeof(R) == sizeof(double)) f.prepare_factors();
     const int lane = static_cast<int>(__lane_id());
     const bool lead = (lane & (G - 1)) == 0;
     for (;;) {
