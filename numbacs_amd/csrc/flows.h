@@ -167,6 +167,21 @@ __device__ __forceinline__ double deg_to_rad(double deg) {
 // (the spherical RHS takes one cosine per evaluation; the library's took ~50
 // instructions of it).  Valid for |x| <= pi; larger arguments (no latitude)
 // take the library cosine.
+// fma(a, b, c) for a coefficient c, taken as a scalar-register operand of
+// v_fma_f64.  Written as fma(), the compiler picks v_fmac_f64, whose addend
+// is also its destination, and first copies each 64-bit coefficient into
+// vector registers: two extra vector instructions per Horner step.  The
+// result is the same fused multiply-add.
+__device__ __forceinline__ double fma_coef(double a, double b, double c) {
+#if __HIP_DEVICE_COMPILE__
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+#else
+    return fma(a, b, c);
+#endif
+}
+
 __device__ __forceinline__ double cos_lat(double x) {
     constexpr double pio2_hi = 1.5707963267948966;      // double nearest pi/2
     constexpr double pio2_lo = 6.123233995736766e-17;   // pi/2 - pio2_hi
@@ -175,17 +190,17 @@ __device__ __forceinline__ double cos_lat(double x) {
     const double u = (pio2_hi - ax) + pio2_lo;
     const double s = u * u;
     double p = 1.0 / 15511210043330985984000000.0;      // 1/25!
-    p = fma(p, s, -1.0 / 25852016738884976640000.0);    // -1/23!
-    p = fma(p, s, 1.0 / 51090942171709440000.0);        // 1/21!
-    p = fma(p, s, -1.0 / 121645100408832000.0);         // -1/19!
-    p = fma(p, s, 1.0 / 355687428096000.0);             // 1/17!
-    p = fma(p, s, -1.0 / 1307674368000.0);              // -1/15!
-    p = fma(p, s, 1.0 / 6227020800.0);                  // 1/13!
-    p = fma(p, s, -1.0 / 39916800.0);                   // -1/11!
-    p = fma(p, s, 1.0 / 362880.0);                      // 1/9!
-    p = fma(p, s, -1.0 / 5040.0);                       // -1/7!
-    p = fma(p, s, 1.0 / 120.0);                         // 1/5!
-    p = fma(p, s, -1.0 / 6.0);                          // -1/3!
+    p = fma_coef(p, s, -1.0 / 25852016738884976640000.0);    // -1/23!
+    p = fma_coef(p, s, 1.0 / 51090942171709440000.0);        // 1/21!
+    p = fma_coef(p, s, -1.0 / 121645100408832000.0);         // -1/19!
+    p = fma_coef(p, s, 1.0 / 355687428096000.0);             // 1/17!
+    p = fma_coef(p, s, -1.0 / 1307674368000.0);              // -1/15!
+    p = fma_coef(p, s, 1.0 / 6227020800.0);                  // 1/13!
+    p = fma_coef(p, s, -1.0 / 39916800.0);                   // -1/11!
+    p = fma_coef(p, s, 1.0 / 362880.0);                      // 1/9!
+    p = fma_coef(p, s, -1.0 / 5040.0);                       // -1/7!
+    p = fma_coef(p, s, 1.0 / 120.0);                         // 1/5!
+    p = fma_coef(p, s, -1.0 / 6.0);                          // -1/3!
     return fma(u * s, p, u);
 }
 
