@@ -278,6 +278,21 @@ int nbx_ridges(const double* f, const double* eigvec_max, const double* x, int n
                double percentile, int min_ridge_pts, double* pts, int64_t* offsets,
                int64_t* nridges_host, int64_t* npts_host, nbx_stream stream);
 
+/* ftle_ridges (ridges.py:186-229) without waiting on the stream: the same
+ * grouping and outputs as nbx_ridges, but the counts go to DEVICE memory,
+ * counts[0] = ridges kept, counts[1] = points kept, counts[2] = ridge points
+ * before the min_ridge_pts filter (3 int64), valid once the stream's work
+ * is done; nothing is read back, so a caller can queue the next field's
+ * flow map behind it or capture it in a graph after a first call (which
+ * sizes the stream's workspace).  Without a percentile threshold
+ * (percentile = 0) the grouping is sized for the whole grid instead of the
+ * ridge-point count.  counts[0] = -1 would report ridge points beyond the
+ * grouping capacity (the capacity bounds them). */
+int nbx_ridges_async(const double* f, const double* eigvec_max, const double* x, int nx,
+                     const double* y, int ny, double dx, double dy, double sdd_thresh,
+                     double percentile, int min_ridge_pts, double* pts, int64_t* offsets,
+                     int64_t* counts, nbx_stream stream);
+
 /* nbx_ridges without allocating: no counterpart in the reference.  For
  * callers that cannot allocate device memory (code under HIP graph capture,
  * serving loops, harnesses that forbid hipMalloc).  Same arguments, results

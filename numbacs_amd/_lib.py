@@ -82,6 +82,8 @@ SIGNATURES = {
                              _c_p, _c_p]),
     "nbx_ridges": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_d, _c_d, _c_i, _c_p,
                           _c_p, _c_p, _c_p, _c_p]),
+    "nbx_ridges_async": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_d, _c_d, _c_i,
+                                _c_p, _c_p, _c_p, _c_p]),
     "nbx_ridges_noalloc": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_d, _c_d, _c_i,
                                   _c_p, _c_p, _c_p, _c_p, _c_p]),
     "nbx_ordered_ridges": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_p, _c_i, _c_d, _c_d, _c_i, _c_d,

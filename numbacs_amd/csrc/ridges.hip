@@ -819,12 +819,37 @@ int read_counts_pageable(const int32_t* dev, int count, int32_t* host, hipStream
     return 0;
 }
 
+// Counts of the asynchronous form: (ridges kept, points kept, ridge points
+// before the length filter); -1 ridges if the points exceeded the grouping
+// capacity m (not reachable: m bounds them, see ridges_run).
+// `grouped` = 0 when the grouping stage did not run (capacity 0): its totals
+// are then not this call's.
+__global__ void k_counts_out(const int32_t* __restrict__ total, int64_t m, int grouped,
+                             int64_t* __restrict__ counts) {
+    if (threadIdx.x == 0) {
+        const int32_t all = total[2];
+        const int32_t kept = grouped ? total[0] : 0, ridges = grouped ? total[1] : 0;
+        counts[0] = all > m ? -1 : ridges;
+        counts[1] = kept;
+        counts[2] = all;
+    }
+}
+
+// (a kernel, not a memset: the call may be captured in a graph)
+__global__ void k_counts_clear(int64_t* __restrict__ counts) {
+    if (threadIdx.x < 3) counts[threadIdx.x] = 0;
+}
+
 // ftle_ridges on a scratch arena of at least RidgeLayout::bytes(nx*ny)
-// bytes at `arena`; n = nx*ny > 0 and the arguments already checked.
+// bytes at `arena`; n = nx*ny > 0 and the arguments already checked.  With
+// counts_dev set the counts go to device memory and nothing waits on the
+// stream (then the grouping capacity without a percentile is the grid size:
+// no count is read back).
 int ridges_run(const double* f, const double* evec, const double* x, int nx, const double* y,
                int ny, double dx, double dy, double sdd_thresh, double percentile,
                int min_ridge_pts, double* pts, int64_t* offsets, int64_t* nridges_host,
-               int64_t* npts_host, char* arena, bool pinned_counts, hipStream_t s) {
+               int64_t* npts_host, char* arena, bool pinned_counts, hipStream_t s,
+               int64_t* counts_dev = nullptr) {
     const int64_t n = static_cast<int64_t>(nx) * ny;
     auto read = [&](const int32_t* dev, int count, int32_t* host) {
         return pinned_counts ? read_counts(dev, count, host, s)
@@ -867,6 +892,8 @@ int ridges_run(const double* f, const double* evec, const double* x, int nx, con
         int64_t klo = static_cast<int64_t>(floor(vi));
         if (vi >= static_cast<double>(n - 1)) klo = n - 1;
         m = n - 1 - klo;
+    } else if (counts_dev) {
+        m = n;
     } else {
         int32_t cnt = 0;
         if (int rc = read(count, 1, &cnt)) return rc;
@@ -875,6 +902,12 @@ int ridges_run(const double* f, const double* evec, const double* x, int nx, con
     if (m == 0) {
         k_offsets_zero<<<1, 1, 0, s>>>(offsets);
         NBX_LAUNCHED("offsets_zero");
+        if (counts_dev) {
+            // no ridge point passes (the grouping totals were not written)
+            k_counts_out<<<1, 64, 0, s>>>(total, 0, 0, counts_dev);
+            NBX_LAUNCHED("counts_out");
+            return 0;
+        }
         NBX_HIP(hipStreamSynchronize(s));
         return 0;
     }
@@ -902,6 +935,11 @@ int ridges_run(const double* f, const double* evec, const double* x, int nx, con
                                         L.out_off_idx, L.tmp2, totals, dense, count, m,
                                         reinterpret_cast<double2*>(pts), offsets);
     NBX_LAUNCHED("group_scatter");
+    if (counts_dev) {
+        k_counts_out<<<1, 64, 0, s>>>(total, m, 1, counts_dev);
+        NBX_LAUNCHED("counts_out");
+        return 0;
+    }
     int32_t h[3] = {0, 0, 0};  // points kept, ridges kept, ridge points (adjacent)
     if (int rc = read(total, 3, h)) return rc;
     // the grouping kernels clip to the capacity m; the percentile bound
@@ -979,6 +1017,27 @@ int nbx_ridges(const double* f, const double* evec, const double* x, int nx, con
     if (ws.error()) return ws.error();
     return ridges_run(f, evec, x, nx, y, ny, dx, dy, sdd_thresh, percentile, min_ridge_pts, pts,
                       offsets, nridges_host, npts_host, ws.get(), true, s);
+}
+
+int nbx_ridges_async(const double* f, const double* evec, const double* x, int nx,
+                     const double* y, int ny, double dx, double dy, double sdd_thresh,
+                     double percentile, int min_ridge_pts, double* pts, int64_t* offsets,
+                     int64_t* counts, nbx_stream stream) {
+    if (int rc = check_grid(nx, ny)) return rc;
+    NBX_REQUIRE(counts != nullptr, "NULL counts");
+    const int64_t n = static_cast<int64_t>(nx) * ny;
+    hipStream_t s = as_stream(stream);
+    if (n == 0) {
+        k_counts_clear<<<1, 64, 0, s>>>(counts);
+        NBX_LAUNCHED("counts_clear");
+        return 0;
+    }
+    NBX_REQUIRE(f && evec && x && y && pts && offsets, "NULL array");
+    Workspace ws(s, 1, RidgeLayout::bytes(n));
+    if (ws.error()) return ws.error();
+    int64_t unused_r = 0, unused_p = 0;
+    return ridges_run(f, evec, x, nx, y, ny, dx, dy, sdd_thresh, percentile, min_ridge_pts, pts,
+                      offsets, &unused_r, &unused_p, ws.get(), true, s, counts);
 }
 
 int nbx_ridges_noalloc(const double* f, const double* evec, const double* x, int nx,

@@ -62,6 +62,32 @@ def ftle_ridges_device(fd, evd, xd, yd, nx, ny, dx, dy, sdd_thresh=0.0, percenti
     return pts, offsets, nr.value, npts.value
 
 
+def ftle_ridges_device_async(fd, evd, xd, yd, nx, ny, dx, dy, sdd_thresh=0.0, percentile=0,
+                             min_ridge_pts=3, pts=None, offsets=None, counts=None):
+    """Device form that does not wait on the stream: returns (pts, offsets,
+    counts), counts a device int64 tensor (ridges, points kept, ridge points
+    before the length filter) valid once the current stream's work is done.
+    pts, offsets and counts may be given (capacities nx*ny x 2, nx*ny + 1
+    and 3) so that a caller can reuse them from call to call."""
+    n = max(nx * ny, 1)
+    pts = _lib.empty((n, 2)) if pts is None else pts
+    offsets = _lib.empty((nx * ny + 1,), "i64") if offsets is None else offsets
+    counts = _lib.empty((3,), "i64") if counts is None else counts
+    if pts.numel() < 2 * n or offsets.numel() < nx * ny + 1 or counts.numel() < 3:
+        raise ValueError("output buffers too small for the grid")
+    t = _lib.torch()
+    for buf, dt in ((pts, t.float64), (offsets, t.int64), (counts, t.int64)):
+        if buf.dtype != dt or not buf.is_contiguous() or not buf.is_cuda:
+            raise ValueError("outputs must be contiguous device tensors: pts float64, "
+                             "offsets and counts int64")
+    _lib.check(_lib.load().nbx_ridges_async(_lib.ptr(fd), _lib.ptr(evd), _lib.ptr(xd), nx,
+                                            _lib.ptr(yd), ny, dx, dy, float(sdd_thresh),
+                                            float(percentile), int(min_ridge_pts),
+                                            _lib.ptr(pts), _lib.ptr(offsets), _lib.ptr(counts),
+                                            _lib.stream_ptr()))
+    return pts, offsets, counts
+
+
 def ftle_ridge_pts(f, eigvec_max, x, y, sdd_thresh=0.0, percentile=0):
     """Sub-pixel FTLE ridge points (k, 2) in raveled grid order: points with
     f > percentile(f) (0 when percentile == 0) where the derivative of f

@@ -537,6 +537,85 @@ def test_ridges_two_streams_concurrently(nb):
             assert len(r) == len(seq[k]) and all(np.array_equal(a, b) for a, b in zip(r, seq[k]))
 
 
+def _split_async(pts, off, cnt):
+    from numbacs_amd import _lib
+
+    nr, nkept, nall = (int(v) for v in _lib.to_host(cnt))
+    P, O = _lib.to_host(pts[:nkept]), _lib.to_host(off[:nr + 1])
+    return [P[O[k]:O[k + 1]] for k in range(nr)], nall
+
+
+@pytest.mark.parametrize("percentile", [0, 90, 100])
+def test_ridges_async_matches_sync(nb, percentile):
+    """nbx_ridges_async (counts left in device memory, nothing waits)
+    against ftle_ridges and the oracle: the same ridges point for point, and
+    counts[2] is the oracle's ridge-point count.  Without a percentile its
+    grouping is sized by the grid (53,803 points: 65,536 keys sorted) rather
+    than by the count read back; at 100 nothing passes."""
+    import torch
+
+    from numbacs_amd import _lib
+    from numbacs_amd.extraction import ftle_ridges
+    from numbacs_amd.extraction.ridges import ftle_ridges_device_async
+
+    f, ev, x, y = _field_and_evec(311, 173, 7)
+    nx, ny = f.shape
+    fd, evd, xd, yd = (_lib.to_dev(a) for a in (f, ev, x, y))
+    cnt = _lib.zeros((3,), "i64") - 7  # stale values the call must overwrite
+    out = ftle_ridges_device_async(fd, evd, xd, yd, nx, ny, float(x[1] - x[0]),
+                                   float(y[1] - y[0]), 0.0, percentile, 3, counts=cnt)
+    torch.cuda.synchronize()
+    got, nall = _split_async(*out)
+    want = ftle_ridges(f, ev, x, y, percentile=percentile, min_ridge_pts=3)
+    ref = oracle.ridges(f, ev, x, y, percentile_=percentile, min_ridge_pts=3)
+    assert nall == len(oracle.ridge_pts(f, ev, x, y, percentile_=percentile))
+    assert len(got) == len(want) == len(ref)
+    assert (len(got) == 0) == (percentile == 100)
+    for a, b, c in zip(got, want, ref):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_ridges_async_pipelined_on_a_second_stream(nb):
+    """A field pipeline: each field is written on the main stream and its
+    ridges queued on a second stream behind an event, while the main stream
+    goes on to the next field (the field buffers reused only after the
+    previous field's ridges are done); every field's ridges equal
+    ftle_ridges' on that field."""
+    import torch
+
+    from numbacs_amd import _lib
+    from numbacs_amd.extraction import ftle_ridges
+    from numbacs_amd.extraction.ridges import ftle_ridges_device_async
+
+    fields = [_field_and_evec(301, 211, s) for s in (1, 2, 3, 4)]
+    x, y = fields[0][2], fields[0][3]
+    nx, ny = len(x), len(y)
+    want = [ftle_ridges(f, ev, x, y, percentile=90) for f, ev, _, _ in fields]
+    assert all(len(w) > 3 for w in want)
+    xd, yd = _lib.to_dev(x), _lib.to_dev(y)
+    fbuf, evbuf = _lib.empty((nx, ny)), _lib.empty((nx, ny, 2))
+    outs = [(_lib.empty((nx * ny, 2)), _lib.empty((nx * ny + 1,), "i64"),
+             _lib.empty((3,), "i64")) for _ in fields]
+    main, side = torch.cuda.current_stream(), torch.cuda.Stream()
+    done = None
+    for k, (f, ev, _, _) in enumerate(fields):
+        if done is not None:
+            main.wait_event(done)
+        fbuf.copy_(torch.from_numpy(f))
+        evbuf.copy_(torch.from_numpy(ev))
+        ready = main.record_event()
+        with torch.cuda.stream(side):
+            side.wait_event(ready)
+            ftle_ridges_device_async(fbuf, evbuf, xd, yd, nx, ny, float(x[1] - x[0]),
+                                     float(y[1] - y[0]), 0.0, 90, 3, *outs[k])
+            done = side.record_event()
+    torch.cuda.synchronize()
+    for k in range(len(fields)):
+        got, _ = _split_async(*outs[k])
+        assert len(got) == len(want[k])
+        assert all(np.array_equal(a, b) for a, b in zip(got, want[k]))
+
+
 def test_ridges_empty_and_degenerate(nb):
     from numbacs_amd.extraction import ftle_ridge_pts, ftle_ridges
 
