@@ -590,47 +590,80 @@ int launch_solve(const Flow& f, const Src& src, const SolveArgs& a, const Launch
 // scattered bucket by bucket: order[k] is the tile launched k-th.  Within a
 // bucket the order is whatever the atomics give; the launch order never
 // changes a particle's result.
-__global__ void k_tile_order_zero(unsigned* __restrict__ count) {
-    if (threadIdx.x < 128) count[threadIdx.x] = 0u;  // counts and cursors
-}
+// Edge-first order: a counting sort of the tiles by bucket (distance of the
+// tile's centre to the nearest edge of the velocity box, in grid cells,
+// clipped to 0..63).  Each block histograms its tiles in LDS and stores its
+// 64 counts; the scatter derives every block's offsets from those counts
+// (no global atomics: nearly every interior tile falls in the last bucket,
+// and a shared global counter serialised them).
+constexpr int kOrderThreads = 256;
 
 template <class Flow>
-__global__ void k_tile_order_keys(Flow f, GridSource src, int64_t tiles, int* __restrict__ bucket,
-                                  unsigned* __restrict__ count) {
+__global__ __launch_bounds__(kOrderThreads) void k_tile_order_keys(
+    Flow f, GridSource src, int64_t tiles, int* __restrict__ bucket,
+    unsigned* __restrict__ block_count) {
+    __shared__ unsigned hist[64];
+    if (threadIdx.x < 64) hist[threadIdx.x] = 0u;
+    __syncthreads();
     const int64_t u = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (u >= tiles) return;
-    const int64_t T = u >> 2, q = u & 3, ntj = (src.ny + 7) / 8;
-    const int64_t i0 = (T / ntj) * 8 + (q >> 1) * 4, j0 = (T % ntj) * 8 + (q & 1) * 4;
-    int b = 63;
-    if (i0 < src.nx && j0 < src.ny) {
-        const int64_t i = i0 + 2 < src.nx ? i0 + 2 : src.nx - 1;
-        const int64_t j = j0 + 2 < src.ny ? j0 + 2 : src.ny - 1;
-        const double x = f.lookup_x(src.xs[i]), y = src.ys[j];
-        const double d = fmin(fmin((x - f.lo[1]) * f.dinv[1], (f.hi[1] - x) * f.dinv[1]),
-                              fmin((y - f.lo[2]) * f.dinv[2], (f.hi[2] - y) * f.dinv[2]));
-        b = d > 0 ? (d < 63 ? static_cast<int>(d) : 63) : 0;
+    if (u < tiles) {
+        const int64_t T = u >> 2, q = u & 3, ntj = (src.ny + 7) / 8;
+        const int64_t i0 = (T / ntj) * 8 + (q >> 1) * 4, j0 = (T % ntj) * 8 + (q & 1) * 4;
+        int b = 63;
+        if (i0 < src.nx && j0 < src.ny) {
+            const int64_t i = i0 + 2 < src.nx ? i0 + 2 : src.nx - 1;
+            const int64_t j = j0 + 2 < src.ny ? j0 + 2 : src.ny - 1;
+            const double x = f.lookup_x(src.xs[i]), y = src.ys[j];
+            const double d = fmin(fmin((x - f.lo[1]) * f.dinv[1], (f.hi[1] - x) * f.dinv[1]),
+                                  fmin((y - f.lo[2]) * f.dinv[2], (f.hi[2] - y) * f.dinv[2]));
+            b = d > 0 ? (d < 63 ? static_cast<int>(d) : 63) : 0;
+        }
+        bucket[u] = b;
+        atomicAdd(&hist[b], 1u);
     }
-    bucket[u] = b;
-    atomicAdd(count + b, 1u);
+    __syncthreads();
+    if (threadIdx.x < 64) block_count[blockIdx.x * 64 + threadIdx.x] = hist[threadIdx.x];
 }
 
-__global__ void k_tile_order_scatter(const int* __restrict__ bucket,
-                                     const unsigned* __restrict__ count,
-                                     unsigned* __restrict__ cursor, int64_t tiles,
-                                     int* __restrict__ order) {
-    __shared__ unsigned base[64];
+__global__ __launch_bounds__(kOrderThreads) void k_tile_order_scatter(
+    const int* __restrict__ bucket, const unsigned* __restrict__ block_count, int nblocks,
+    int64_t tiles, int* __restrict__ order) {
+    // per bucket: the tiles of all blocks (total) and of the blocks before
+    // this one (before), four partial sums per bucket over the block counts
+    __shared__ unsigned total[4][64], before[4][64];
+    __shared__ unsigned base[64], cursor[64];
+    const int bk = threadIdx.x & 63, part = threadIdx.x >> 6;
+    unsigned tot = 0u, bef = 0u;
+    for (int k = part; k < nblocks; k += 4) {
+        const unsigned c = block_count[k * 64 + bk];
+        tot += c;
+        bef += k < static_cast<int>(blockIdx.x) ? c : 0u;
+    }
+    total[part][bk] = tot;
+    before[part][bk] = bef;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        base[bk] = total[0][bk] + total[1][bk] + total[2][bk] + total[3][bk];
+        cursor[bk] = before[0][bk] + before[1][bk] + before[2][bk] + before[3][bk];
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned acc = 0;
+        unsigned acc = 0u;
         for (int b = 0; b < 64; ++b) {
+            const unsigned n = base[b];
             base[b] = acc;
-            acc += count[b];
+            acc += n;
         }
     }
     __syncthreads();
     const int64_t u = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (u >= tiles) return;
     const int b = bucket[u];
-    order[base[b] + atomicAdd(cursor + b, 1u)] = static_cast<int>(u);
+    order[base[b] + atomicAdd(&cursor[b], 1u)] = static_cast<int>(u);
+}
+
+inline int64_t tile_order_blocks(int64_t tiles) {
+    return (tiles + kOrderThreads - 1) / kOrderThreads;
 }
 
 // Enqueue the edge-first order of src's tiles into the stream's workspace
@@ -643,23 +676,25 @@ int tile_order_edge_first(const Flow& f, const GridSource& src, hipStream_t s, W
     if (tiles == 0) return 0;
     NBX_REQUIRE(tiles < (int64_t(1) << 31), "too many particle tiles");
     if (ws.error()) return ws.error();
+    const int64_t nb = tile_order_blocks(tiles);
     char* w = ws.get();
-    unsigned* count = reinterpret_cast<unsigned*>(w);  // 64 counts, then 64 cursors
-    int* bucket = reinterpret_cast<int*>(w + 512);
+    unsigned* block_count = reinterpret_cast<unsigned*>(w);  // 64 per block
+    int* bucket = reinterpret_cast<int*>(w + 256 * nb);
     int* order = bucket + tiles;
-    k_tile_order_zero<<<1, 128, 0, s>>>(count);
-    NBX_LAUNCHED("tile_order_zero");
-    k_tile_order_keys<<<blocks_for(tiles, 256), 256, 0, s>>>(f, src, tiles, bucket, count);
+    k_tile_order_keys<<<static_cast<unsigned>(nb), kOrderThreads, 0, s>>>(f, src, tiles, bucket,
+                                                                          block_count);
     NBX_LAUNCHED("tile_order_keys");
-    k_tile_order_scatter<<<blocks_for(tiles, 256), 256, 0, s>>>(bucket, count, count + 64,
-                                                                  tiles, order);
+    k_tile_order_scatter<<<static_cast<unsigned>(nb), kOrderThreads, 0, s>>>(
+        bucket, block_count, static_cast<int>(nb), tiles, order);
     NBX_LAUNCHED("tile_order_scatter");
     out.order = order;
     return 0;
 }
 
 inline size_t tile_order_bytes(const GridSource& src) {
-    return 512 + 2 * sizeof(int) * static_cast<size_t>(src.slots() / 16);
+    const int64_t tiles = src.slots() / 16;
+    return 256 * static_cast<size_t>(tile_order_blocks(tiles)) +
+           2 * sizeof(int) * static_cast<size_t>(tiles);
 }
 
 template <class R, class Src>
