@@ -597,6 +597,10 @@ int launch_solve(const Flow& f, const Src& src, const SolveArgs& a, const Launch
 // (no global atomics: nearly every interior tile falls in the last bucket,
 // and a shared global counter serialised them).
 constexpr int kOrderThreads = 256;
+#ifndef NBX_ORDER_BLOCK
+#define NBX_ORDER_BLOCK 32
+#endif
+constexpr int64_t kOrderBlock = NBX_ORDER_BLOCK;  // particles per side, a multiple of 8
 
 template <class Flow>
 __global__ __launch_bounds__(kOrderThreads) void k_tile_order_keys(
@@ -611,8 +615,13 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order_keys(
         const int64_t i0 = (T / ntj) * 8 + (q >> 1) * 4, j0 = (T % ntj) * 8 + (q & 1) * 4;
         int b = 63;
         if (i0 < src.nx && j0 < src.ny) {
-            const int64_t i = i0 + 2 < src.nx ? i0 + 2 : src.nx - 1;
-            const int64_t j = j0 + 2 < src.ny ? j0 + 2 : src.ny - 1;
+            // the centre of the tile's block of kOrderBlock x kOrderBlock
+            // particles: whole blocks share a bucket, so each bucket is swept
+            // band by band in tile order (compact working sets in L2)
+            const int64_t ib = (i0 / kOrderBlock) * kOrderBlock + kOrderBlock / 2;
+            const int64_t jb = (j0 / kOrderBlock) * kOrderBlock + kOrderBlock / 2;
+            const int64_t i = ib < src.nx ? ib : src.nx - 1;
+            const int64_t j = jb < src.ny ? jb : src.ny - 1;
             const double x = f.lookup_x(src.xs[i]), y = src.ys[j];
             const double d = fmin(fmin((x - f.lo[1]) * f.dinv[1], (f.hi[1] - x) * f.dinv[1]),
                                   fmin((y - f.lo[2]) * f.dinv[2], (f.hi[2] - y) * f.dinv[2]));
