@@ -118,6 +118,17 @@ __device__ __forceinline__ void bspline_weights(R s, R w[4]) {
     w[3] = s3 * sixth;
 }
 
+// The same weights times 6 (no scaling: the 4-lane gather applies the
+// three axes' 1/6 once, as 1/216 on its y-weight)
+template <class R>
+__device__ __forceinline__ void bspline_weights6(R s, R w[4]) {
+    const R s2 = s * s, s3 = s2 * s, u = R(1) - s;
+    w[0] = u * u * u;
+    w[1] = R(3) * s3 - R(6) * s2 + R(4);
+    w[2] = R(-3) * s3 + R(3) * s2 + R(3) * s + R(1);
+    w[3] = s3;
+}
+
 template <class R>
 __device__ __forceinline__ void bspline_dweights(R s, R dw[4]) {
     const R s2 = s * s, u = R(1) - s;
@@ -438,9 +449,15 @@ struct Spline3 {
         cell(1, qx, ix, s1);
         cell(2, qy, iy, s2);
         R wt[4], wx[4], wy[4];
-        bspline_weights(s0, wt);
-        bspline_weights(s1, wx);
-        bspline_weights(s2, wy);
+        if constexpr (G == 4) {
+            bspline_weights6(s0, wt);
+            bspline_weights6(s1, wx);
+            bspline_weights6(s2, wy);
+        } else {
+            bspline_weights(s0, wt);
+            bspline_weights(s1, wx);
+            bspline_weights(s2, wy);
+        }
         const P* base = C + (static_cast<long>(it) * st + static_cast<long>(ix) * sx + iy);
         // explicit fma: the accumulation order and rounding do not depend on
         // how the compiler contracts each kernel instantiation
@@ -520,7 +537,7 @@ struct Spline3 {
             } else {
                 column_sums(base + g, st, sx, wt, wx, au, av);
             }
-            const R wc = pick4(wy, g);
+            const R wc = pick4(wy, g) * R(1.0 / 216.0);
             au *= wc;
             av *= wc;
         } else {
