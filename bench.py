@@ -54,6 +54,12 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-hbm", action="store_true",
+                    help="gridded workload: take the flow map's HBM bytes from the committed "
+                         "rocprofv3 record instead of measuring them (child runs under "
+                         "rocprofv3 --pmc)")
+    ap.add_argument("--tile-order", choices=["edge", "plain"], default=None,
+                    help="gridded workload: fixed tile order of the FP64 grid launch")
     ap.add_argument("--no-timeseries", action="store_true",
                     help="default workload: leave out the time-series legs")
     ap.add_argument("--launch", default=None, metavar="LANES,BLOCK,PERSIST",
@@ -119,6 +125,8 @@ def bench_gridded(args):
         flowmap_grid_2d_device(W["flow"], t0, T, xd, yd, params, 1e-6, 1e-8, out=fm,
                                status=status, nsteps=nsteps)
 
+    if args.tile_order:
+        autotune.set_order(_lib.NBX_FP64, args.tile_order == "edge")
     if args.launch:
         autotune.set_config(_lib.NBX_FLOW_SPLINE3, _lib.NBX_FP64,
                             tuple(int(v) for v in args.launch.split(",")))
@@ -194,7 +202,13 @@ def bench_gridded(args):
     achieved_gbs = bytes_launch / (fm_ms * 1e-3) / 1e9
     achieved_tflops = flops_launch / (fm_ms * 1e-3) / 1e12
 
-    hbm = hbm_traffic(fm_ms)
+    order_edge = bool(autotune.get_order(_lib.NBX_FP64))
+    hbm = None
+    if not args.no_hbm:
+        print("bench: HBM counters (rocprofv3 child runs)", file=sys.stderr, flush=True)
+        hbm = hbm_traffic_live(fm_ms, tuned, order_edge)
+    if hbm is None:
+        hbm = hbm_traffic(fm_ms)
     rec = {
         "metric": "FTLE grid points/s (gridded-velocity FTLE + ridges, plain FP64 DOP853)",
         "value": P / (wall / args.steps),
@@ -258,6 +272,83 @@ def bench_gridded(args):
         print("bench: time-series legs", file=sys.stderr, flush=True)
         rec["timeseries"] = timeseries_legs(tune=not args.no_tune)
     return rec
+
+
+def _pmc_mean(counter, cmd, match, timeout=180):
+    """Mean per-dispatch value of one rocprofv3 counter over the dispatches
+    whose kernel name contains every string in `match`, from a child run of
+    `cmd` under `rocprofv3 --pmc`: (mean, dispatches), or None when
+    rocprofv3 is absent or the run fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    d = tempfile.mkdtemp(prefix="nbx_pmc_", dir="/tmp")
+    try:
+        r = subprocess.run([exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run",
+                            "--"] + cmd, cwd=ROOT, env=dict(os.environ, TMPDIR="/tmp"),
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode != 0 or not files:
+            return None
+        per = {}
+        with open(files[0]) as fh:
+            for row in csv.DictReader(fh):
+                if all(m in row["Kernel_Name"] for m in match):
+                    key = row["Dispatch_Id"]
+                    per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+        return (sum(per.values()) / len(per), len(per)) if per else None
+    except (subprocess.TimeoutExpired, OSError, ValueError, KeyError):
+        return None
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def hbm_traffic_live(kernel_ms, launch, edge_first):
+    """HBM bytes per launch of the gridded spline flow map, measured now: the
+    flow map's FETCH_SIZE and WRITE_SIZE (KiB per dispatch) from two child
+    runs of this bench under rocprofv3 --pmc at the same launch shape and
+    tile order, each scaled by what the counter reads for a 512 MiB device
+    copy (two more child runs).  None if any run fails."""
+    py = [sys.executable, "-u", os.path.join(ROOT, "bench.py")]
+    fm_cmd = py + ["--steps", "2", "--warmup", "1", "--no-cpu", "--no-timeseries", "--no-hbm",
+                   "--launch", ",".join(str(int(v)) for v in launch),
+                   "--tile-order", "edge" if edge_first else "plain"]
+    cal_cmd = py + ["--workload", "copycal"]
+    match = ("k_flowmap", "Spline3<double")
+    got = {}
+    for key, counter, cmd, m in (("fetch", "FETCH_SIZE", fm_cmd, match),
+                                 ("write", "WRITE_SIZE", fm_cmd, match),
+                                 ("cal_fetch", "FETCH_SIZE", cal_cmd, ("copyBuffer",)),
+                                 ("cal_write", "WRITE_SIZE", cal_cmd, ("copyBuffer",))):
+        print(f"bench: rocprofv3 --pmc {counter} ({'copy calibration' if 'cal' in key else 'flow map'})",
+              file=sys.stderr, flush=True)
+        got[key] = _pmc_mean(counter, cmd, m)
+        if got[key] is None or got[key][0] <= 0:
+            return None
+    copy_bytes = 64 * 1024 * 1024 * 8
+    f_fetch = copy_bytes / (got["cal_fetch"][0] * 1024)
+    f_write = copy_bytes / (got["cal_write"][0] * 1024)
+    rd = got["fetch"][0] * 1024 * f_fetch
+    wr = got["write"][0] * 1024 * f_write
+    b = rd + wr
+    gbs = b / (kernel_ms * 1e-3) / 1e9
+    return {"bytes_per_launch": b, "read_bytes": rd, "write_bytes": wr, "achieved_GBs": gbs,
+            "peak_GBs": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
+            "fetch_size_kib_per_dispatch": got["fetch"][0],
+            "write_size_kib_per_dispatch": got["write"][0],
+            "dispatches": [got["fetch"][1], got["write"][1]],
+            "calibration": {"copy_bytes": copy_bytes, "fetch_factor": f_fetch,
+                            "write_factor": f_write},
+            "source": "measured by this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE child "
+                      "runs of bench.py at the same launch and tile order (bench.py " +
+                      " ".join(fm_cmd[3:]) + "), scaled by a 512 MiB device copy "
+                      "(bench.py --workload copycal)"}
 
 
 def hbm_traffic(kernel_ms):
