@@ -204,7 +204,10 @@ def bench_gridded(args):
 
     order_edge = bool(autotune.get_order(_lib.NBX_FP64))
     hbm = None
-    if not args.no_hbm:
+    # (not when this run is itself under rocprofv3, whose launcher sets
+    # ROCPROF_* variables for its target: no profiler inside a profiler)
+    under_profiler = any(k.startswith("ROCPROF_") for k in os.environ)
+    if not args.no_hbm and not under_profiler:
         print("bench: HBM counters (rocprofv3 child runs)", file=sys.stderr, flush=True)
         hbm = hbm_traffic_live(fm_ms, tuned, order_edge)
     if hbm is None:
