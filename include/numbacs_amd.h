@@ -170,10 +170,11 @@ int nbx_get_launch_config(int kind, int precision, int* lanes_per_particle, int*
 
 /* Launch order of the particle tiles in the spline flows' grid launch
  * (persistent_blocks_per_cu 0), per precision: edge_first 1 (the default)
- * launches the tiles nearest the velocity box's edge first -- particles
- * leaving the box cross the jump to zero velocity, take the most steps and
- * would otherwise form the launch's tail -- edge_first 0 keeps plain tile
- * order.  Results do not depend on it.  Process-wide; the autotuner times
+ * launches the tiles nearest the velocity box's edge first, by the distance
+ * of each tile's 32 x 32-particle block (whole blocks in tile order within a
+ * distance, for L2 locality) -- particles leaving the box cross the jump to
+ * zero velocity, take the most steps and would otherwise form the launch's
+ * tail -- edge_first 0 keeps plain tile order.  Results do not depend on it.  Process-wide; the autotuner times
  * both.  No reference counterpart. */
 int nbx_set_tile_order(int precision, int edge_first);
 int nbx_get_tile_order(int precision, int* edge_first);

@@ -625,6 +625,21 @@ def test_ridges_empty_and_degenerate(nb):
     assert ftle_ridge_pts(f, ev, x, y).shape == (0, 2)
     assert ftle_ridges(f, ev, x, y) == []
     assert ftle_ridges(f, ev, x, y, percentile=90) == []
+    # the asynchronous form: a flat field and an empty grid leave zero counts
+    # (stale values in the counts buffer are overwritten, not kept)
+    import torch
+
+    from numbacs_amd import _lib
+    from numbacs_amd.extraction.ridges import ftle_ridges_device_async
+
+    for nx in (6, 0):
+        fd = _lib.to_dev(f[:nx]) if nx else _lib.empty((0, 5))
+        evd = _lib.to_dev(ev[:nx]) if nx else _lib.empty((0, 5, 2))
+        cnt = _lib.zeros((3,), "i64") - 7
+        ftle_ridges_device_async(fd, evd, _lib.to_dev(x), _lib.to_dev(y), nx, 5, 1.0, 1.0, 0.0,
+                                 90, 3, counts=cnt)
+        torch.cuda.synchronize()
+        assert _lib.to_host(cnt).tolist() == [0, 0, 0]
 
 
 # ----------------------------------------------------- trilinear / ingest --
