@@ -591,8 +591,8 @@ int launch_solve(const Flow& f, const Src& src, const SolveArgs& a, const Launch
 // bucket the order is whatever the atomics give; the launch order never
 // changes a particle's result.
 // Edge-first order: a counting sort of the tiles by bucket (distance of the
-// tile's centre to the nearest edge of the velocity box, in grid cells,
-// clipped to 0..63).  Each block histograms its tiles in LDS and stores its
+// centre of the tile's block of kOrderBlock x kOrderBlock particles to the
+// nearest edge of the velocity box, in grid cells, clipped to 0..63).  Each block histograms its tiles in LDS and stores its
 // 64 counts; the scatter derives every block's offsets from those counts
 // (no global atomics: nearly every interior tile falls in the last bucket,
 // and a shared global counter serialised them).
