@@ -288,7 +288,8 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap(Flow f, 
                                                  int32_t* __restrict__ status,
                                                  int32_t* __restrict__ nsteps) {
     f.prepare();
-    const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const unsigned bid = std::is_same<Src, GridSource>::value ? xcd_block() : blockIdx.x;
+    const int64_t tid = bid * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     int64_t p = tid / G;
     if (p >= P) return;  // whole groups leave together (P counts slots for grids)
     if constexpr (std::is_same<Src, GridSource>::value) {

@@ -80,6 +80,30 @@ struct ReqDev {
     int32_t* nsteps;              // nullable
 };
 
+// Blocks per XCD chunk of the tile launches' block remap (0: no remap).
+#ifndef NBX_XCD_CHUNK
+#define NBX_XCD_CHUNK 64
+#endif
+
+// The work block of this block in a tile launch.  Consecutive blocks go to
+// the eight XCDs in turn; XCD x is given the work of chunks x, x + 8, ... of
+// NBX_XCD_CHUNK consecutive blocks, so the tiles an XCD runs together are
+// neighbours (sharing its own L2) while every XCD still takes an even share
+// of each stretch of the launch order.  A bijection on the blocks of whole
+// groups of 8 chunks; the remaining blocks keep their index.
+__device__ __forceinline__ unsigned xcd_block() {
+    unsigned bid = blockIdx.x;
+#if NBX_XCD_CHUNK > 0
+    constexpr unsigned C = NBX_XCD_CHUNK;
+    const unsigned full = gridDim.x / (8u * C) * (8u * C);
+    if (bid < full) {
+        const unsigned x = bid % 8u, g = bid / 8u;
+        bid = ((g / C) * 8u + x) * C + g % C;
+    }
+#endif
+    return bid;
+}
+
 // Attempted steps per particle before a wavefront hands its tile back.
 constexpr int kChunkSteps = 16;
 
@@ -269,7 +293,7 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap_tiles(
     Flow f, const ReqDev* __restrict__ table, const int64_t* __restrict__ slot_off, int nreq,
     int64_t total_slots, double rtol, double atol, long long nmax) {
     f.prepare();
-    const int64_t tid = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const int64_t tid = xcd_block() * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     const int64_t q = tid / G;
     if (q >= total_slots) return;  // whole groups leave together
     int lo = 0, hi = nreq - 1;
