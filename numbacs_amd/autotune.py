@@ -92,6 +92,24 @@ def tune(kind, precision, run, size, reps=3, persistent=True, use_cache=True, ca
         t.cuda.synchronize()
         times[",".join(map(str, cfg))] = 1e3 * (time.perf_counter() - t0) / reps
     best_s = min(times, key=times.get)
+    # shapes within 5% of the best are timed again, in three interleaved
+    # rounds (clock drift then hits all of them alike), by their median
+    finalists = [k for k, v in times.items() if v <= 1.05 * times[best_s]]
+    if len(finalists) > 1:
+        rounds = {k: [] for k in finalists}
+        for _ in range(3):
+            for k in finalists:
+                set_config(kind, precision, tuple(int(v) for v in k.split(",")))
+                run()
+                t.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    run()
+                t.cuda.synchronize()
+                rounds[k].append(1e3 * (time.perf_counter() - t0) / reps)
+        for k in finalists:
+            times[k] = sorted(rounds[k])[1]
+        best_s = min(finalists, key=lambda k: times[k])
     best = tuple(int(v) for v in best_s.split(","))
     set_config(kind, precision, best if best else old)
     cache[key] = {"cfg": list(best), "times": times}
