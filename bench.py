@@ -277,7 +277,7 @@ def bench_gridded(args):
     return rec
 
 
-def _pmc_mean(counter, cmd, match, timeout=180):
+def _pmc_mean(counter, cmd, match, timeout=120):
     """Mean per-dispatch value of one rocprofv3 counter over the dispatches
     whose kernel name contains every string in `match`, from a child run of
     `cmd` under `rocprofv3 --pmc`: (mean, dispatches), or None when
