@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap(Flow f, 
                                                  int32_t* __restrict__ status,
                                                  int32_t* __restrict__ nsteps) {
     f.prepare();
-    const unsigned bid = xcd_block();
+    const unsigned bid = xcd_block<Flow>();
     const int64_t tid = bid * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     int64_t p = tid / G;
     if (p >= P) return;  // whole groups leave together (P counts slots for grids)

@@ -80,25 +80,39 @@ struct ReqDev {
     int32_t* nsteps;              // nullable
 };
 
-// Blocks per XCD chunk of the tile launches' block remap (0: no remap).
+// Blocks per chunk of the spline tile launches' block remap (0: no remap).
 #ifndef NBX_XCD_CHUNK
 #define NBX_XCD_CHUNK 64
 #endif
 
-// The work block of this block in a tile launch.  Consecutive blocks go to
-// the eight XCDs in turn; XCD x is given the work of chunks x, x + 8, ... of
-// NBX_XCD_CHUNK consecutive blocks, so the tiles an XCD runs together are
-// neighbours (sharing its own L2) while every XCD still takes an even share
-// of each stretch of the launch order.  A bijection on the blocks of whole
-// groups of 8 chunks; the remaining blocks keep their index.
+template <class T>
+struct IsSpline3 : std::false_type {};
+template <class R, int G, bool L, bool S>
+struct IsSpline3<Spline3<R, G, L, S>> : std::true_type {};
+
+// The work block of this block in a tile launch over a spline flow.
+// Consecutive blocks go to the eight XCDs in turn; XCD x is given the work
+// of chunks x, x + 8, ... of NBX_XCD_CHUNK consecutive blocks, so the tiles
+// an XCD runs together are neighbours (their stencils share its own L2)
+// while every XCD still takes an even share of each stretch of the launch
+// order.  Only when every XCD gets at least 10 chunks (fewer, larger chunks
+// are regions whose long tiles pile up on one XCD), and only for spline
+// flows, where it was measured faster (the trilinear flow map was 15%
+// slower with it).  A bijection on the blocks of whole groups of 8 chunks;
+// the remaining blocks keep their index.
+template <class Flow>
 __device__ __forceinline__ unsigned xcd_block() {
     unsigned bid = blockIdx.x;
 #if NBX_XCD_CHUNK > 0
-    constexpr unsigned C = NBX_XCD_CHUNK;
-    const unsigned full = gridDim.x / (8u * C) * (8u * C);
-    if (bid < full) {
-        const unsigned x = bid % 8u, g = bid / 8u;
-        bid = ((g / C) * 8u + x) * C + g % C;
+    if constexpr (IsSpline3<Flow>::value) {
+        constexpr unsigned C = NBX_XCD_CHUNK;
+        if (gridDim.x >= 80u * C) {
+            const unsigned full = gridDim.x / (8u * C) * (8u * C);
+            if (bid < full) {
+                const unsigned x = bid % 8u, g = bid / 8u;
+                bid = ((g / C) * 8u + x) * C + g % C;
+            }
+        }
     }
 #endif
     return bid;
@@ -293,7 +307,7 @@ __global__ __launch_bounds__(256, NBX_FLOWMAP_MIN_WAVES) void k_flowmap_tiles(
     Flow f, const ReqDev* __restrict__ table, const int64_t* __restrict__ slot_off, int nreq,
     int64_t total_slots, double rtol, double atol, long long nmax) {
     f.prepare();
-    const int64_t tid = xcd_block() * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const int64_t tid = xcd_block<Flow>() * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     const int64_t q = tid / G;
     if (q >= total_slots) return;  // whole groups leave together
     int lo = 0, hi = nreq - 1;
