@@ -282,7 +282,6 @@ def _pmc_mean(counter, cmd, match, timeout=120):
     whose kernel name contains every string in `match`, from a child run of
     `cmd` under `rocprofv3 --pmc`: (mean, dispatches), or None when
     rocprofv3 is absent or the run fails."""
-    import csv
     import glob
     import shutil
     import subprocess
@@ -299,17 +298,26 @@ def _pmc_mean(counter, cmd, match, timeout=120):
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         if r.returncode != 0 or not files:
             return None
-        per = {}
-        with open(files[0]) as fh:
-            for row in csv.DictReader(fh):
-                if all(m in row["Kernel_Name"] for m in match):
-                    key = row["Dispatch_Id"]
-                    per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
-        return (sum(per.values()) / len(per), len(per)) if per else None
+        return pmc_dispatch_mean(files[0], match)
     except (subprocess.TimeoutExpired, OSError, ValueError, KeyError):
         return None
     finally:
         shutil.rmtree(d, ignore_errors=True)
+
+
+def pmc_dispatch_mean(path, match):
+    """(mean per dispatch, dispatches) of a rocprofv3 counter-collection CSV
+    over the dispatches whose kernel name contains every string in `match`
+    (a counter's rows of one dispatch are summed), or None if none match."""
+    import csv
+
+    per = {}
+    with open(path) as fh:
+        for row in csv.DictReader(fh):
+            if all(m in row["Kernel_Name"] for m in match):
+                key = row["Dispatch_Id"]
+                per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+    return (sum(per.values()) / len(per), len(per)) if per else None
 
 
 def hbm_traffic_live(kernel_ms, launch, edge_first):

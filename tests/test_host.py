@@ -59,3 +59,28 @@ def test_flow_handle_not_picklable():
     f = get_predefined_flow("double_gyre", return_default_params=False, return_domain=False)
     with pytest.raises(TypeError):
         pickle.dumps(f)
+
+
+def test_pmc_dispatch_mean_parses_counter_csv(tmp_path):
+    """bench.py's reading of a rocprofv3 counter-collection CSV (the flow
+    map's FETCH_SIZE / WRITE_SIZE per dispatch): rows of one dispatch are
+    summed (a counter collected per XCD or channel), dispatches averaged,
+    other kernels ignored; no match gives None."""
+    import importlib.util
+    import os
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("nbx_bench", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    rows = [("1", "void nbx::k_flowmap<nbx::Spline3<double, 4, false, false>, double, 4>", 100.0),
+            ("1", "void nbx::k_flowmap<nbx::Spline3<double, 4, false, false>, double, 4>", 20.0),
+            ("2", "void nbx::k_ftle_strip<true>(...)", 999.0),
+            ("3", "void nbx::k_flowmap<nbx::Spline3<double, 4, false, false>, double, 4>", 60.0)]
+    path = tmp_path / "run_counter_collection.csv"
+    with open(path, "w") as fh:
+        fh.write("Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n")
+        for d, k, v in rows:
+            fh.write(f'{d},"{k}",FETCH_SIZE,{v}\n')
+    assert bench.pmc_dispatch_mean(str(path), ("k_flowmap", "Spline3<double")) == (90.0, 2)
+    assert bench.pmc_dispatch_mean(str(path), ("copyBuffer",)) is None
